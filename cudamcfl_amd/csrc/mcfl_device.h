@@ -1,0 +1,118 @@
+/*
+ * mcfl_device.h -- device-side data layout of the transport launch, shared by
+ * the HIP kernel (mcfl_transport.hip) and the host code that fills it.
+ *
+ * A launch runs one flattened photon index space [0, total) over a single
+ * voxel medium.  The index space is cut either into explicit jobs
+ * (MCFL_WORK_JOBS: one excitation simulation, a multi-source batch, the
+ * current epoch of several served requests), into voxel emission jobs
+ * (MCFL_WORK_VOXELS: the fluorescence stage, one job per emitting voxel), or
+ * into the reference's per-lane streams (MCFL_WORK_LANES).
+ */
+#ifndef MCFL_DEVICE_H
+#define MCFL_DEVICE_H
+
+#include <stdint.h>
+
+#define MCFL_MAX_MATERIALS 128   /* labels are 8-bit; the reference caps bulks at 100 (CUDAMCFL.h:24) */
+#define MCFL_MAX_JOBS 64         /* explicit jobs per launch (per-job totals live in LDS) */
+#define MCFL_NTOT 16             /* totals per job, see enum mcfl_total in mcfl.h */
+#define MCFL_MAX_STEPS_PER_PHOTON (1u << 28)
+
+enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2 };
+enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2 };
+enum {
+  MCFL_TOT_LAUNCHED_W = 0,
+  MCFL_TOT_R,
+  MCFL_TOT_T,
+  MCFL_TOT_A,
+  MCFL_TOT_ROUL_GAIN,
+  MCFL_TOT_ROUL_LOSS,
+  MCFL_TOT_ZERO_KILL,
+  MCFL_TOT_PHOTONS,
+  MCFL_TOT_STEPS,
+  MCFL_TOT_ABORTED
+};
+
+/* Per-material constants (32 bytes, LDS resident).  twog == 0 <=> g == 0. */
+struct MatDev {
+  float mutr;     /* 1/(mua_base + mus), FLT_MAX for glass */
+  float mua_mutr; /* mua*mutr: absorbed fraction per interaction */
+  float n;        /* refractive index */
+  float onemg;    /* 1 - g */
+  float onemg2;   /* 1 - g^2 */
+  float onepg2;   /* 1 + g^2 */
+  float twog;     /* 2g */
+  float inv2g;    /* 1/(2g), 0 when g == 0 */
+};
+
+struct MediumDev {
+  const uint8_t* labels; /* nx*ny*nz, x fastest */
+  int32_t nx, ny, nz;
+  uint32_t last; /* n_bulks + 1 */
+  float esp, gs, halfx, halfy;
+};
+
+struct DetDev {
+  float dx, dy, x0, y0, size_x, size_y;
+  int32_t nx, ny;
+  uint32_t npix; /* nx*(ny+1) */
+  /* temporal detectors */
+  uint32_t t_ndx, t_ndy, t_rort;
+  float t_r;
+  uint64_t t_bins, t_max;
+  const float* t_posx;
+  const float* t_posy;
+};
+
+/* Tally flags */
+enum { MCFL_F_FHD = 1, MCFL_F_AXYZ = 2, MCFL_F_SQ = 4, MCFL_F_TEMP = 8, MCFL_F_SCALED = 16 };
+
+struct JobDev {
+  uint64_t begin;  /* first flattened photon index */
+  uint64_t count;  /* photons */
+  uint64_t key;    /* photon-stream key */
+  int32_t src_kind;
+  float xi, yi, zi, src_r;
+  uint32_t start_weight;
+  uint32_t mat_base; /* first entry of this job's material table in the launch table */
+  uint32_t flags;
+  double exit_scale;
+  unsigned long long* Rd;
+  unsigned long long* Tt;
+  unsigned long long* fhd;
+  unsigned long long* axyz;
+  unsigned long long* time;
+  unsigned long long* Rd2;
+  unsigned long long* Tt2;
+  unsigned long long* tot; /* MCFL_NTOT counters */
+  DetDev det;
+};
+
+struct LaunchArgs {
+  MediumDev med;
+  const MatDev* mats; /* n_mats entries (all jobs' tables, concatenated) */
+  uint32_t n_mats;
+  int32_t work; /* MCFL_WORK_* */
+  uint64_t total; /* size of the flattened index space */
+  unsigned long long* counter; /* global work counter (zeroed before launch) */
+  const uint32_t* a_tab; /* photon-stream multipliers */
+  uint32_t a_mask;       /* table length - 1 (power of two) */
+  /* MCFL_WORK_JOBS */
+  const JobDev* jobs;
+  uint32_t njobs;
+  /* MCFL_WORK_VOXELS: job 0 is the template; voxel j emits per_job photons */
+  const uint32_t* vox;
+  const double* vscale;
+  uint64_t per_job;
+  uint64_t vseed;
+  /* MCFL_WORK_LANES: lane i runs q (+1 if i < rem) photons on (xs[i], as[i]) */
+  unsigned long long* xs;
+  const uint32_t* as;
+  uint64_t lane_q, lane_rem;
+  /* launch geometry */
+  uint32_t n_waves;
+  uint32_t chunk_max;
+};
+
+#endif

@@ -1,0 +1,140 @@
+/*
+ * mcfl_fmath.h -- deterministic FP32 arithmetic shared by the HIP transport
+ * kernel and by host C code.
+ *
+ * The reference evaluates its transport with CUDA fast-math intrinsics
+ * (__logf, __sincosf, __fdividef; CUDAMCFLtransport.cu:290,596-603 and
+ * Compile750:1 "--use_fast_math").  Those are not reproducible off NVIDIA
+ * hardware, so this build defines its own transcendental functions from
+ * published methods, built only from IEEE-754 operations that every
+ * conforming implementation rounds identically (+, -, *, fused multiply-add,
+ * correctly rounded division and square root, exact integer bit operations).
+ * Compiled with -ffp-contract=off on both sides, a kernel and a host program
+ * that use this header execute bit-identical FP32 arithmetic, so integer
+ * tallies of a fixed photon set agree exactly between CPU and GPU.
+ *
+ *  - logarithm: exponent/mantissa split with the mantissa folded into
+ *    [sqrt(1/2), sqrt(2)), log1p(f) = f + f^2 R(f) with R a degree-8
+ *    polynomial fitted by least squares on Chebyshev nodes (near-minimax,
+ *    Remez-style approximation), Cody-Waite two-constant ln2 split.
+ *    Max relative error about 2 ulp on (0, 1].
+ *  - sine/cosine of 2*pi*t ("turns"): exact quadrant reduction 4t - rint(4t)
+ *    (no large-argument reduction is needed because the argument is a
+ *    fraction of a turn), then odd/even degree-9/8 polynomials in the
+ *    reduced variable fitted the same way.  Max error about 1 ulp.
+ *
+ * This header is valid C99 and HIP C++.
+ */
+#ifndef MCFL_FMATH_H
+#define MCFL_FMATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define MCFL_HD static __host__ __device__ __forceinline__
+#else
+#include <string.h>
+#define MCFL_HD static inline
+#endif
+
+MCFL_HD uint32_t mcfl_fbits(float f) {
+#if defined(__HIPCC__) || defined(__HIP__)
+  return __builtin_bit_cast(uint32_t, f);
+#else
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+#endif
+}
+
+MCFL_HD float mcfl_bitsf(uint32_t u) {
+#if defined(__HIPCC__) || defined(__HIP__)
+  return __builtin_bit_cast(float, u);
+#else
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+#endif
+}
+
+MCFL_HD float mcfl_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+MCFL_HD float mcfl_sqrt(float x) { return __builtin_sqrtf(x); }
+MCFL_HD float mcfl_rint(float x) { return __builtin_rintf(x); }
+MCFL_HD float mcfl_ceil(float x) { return __builtin_ceilf(x); }
+MCFL_HD float mcfl_fabs(float x) { return __builtin_fabsf(x); }
+MCFL_HD float mcfl_copysign(float m, float s) { return __builtin_copysignf(m, s); }
+MCFL_HD float mcfl_fmax(float a, float b) { return a > b ? a : b; }
+
+/* Natural logarithm for positive normal floats. */
+MCFL_HD float mcfl_log(float u) {
+  uint32_t b = mcfl_fbits(u);
+  int e = (int)(b >> 23) - 127;
+  uint32_t mb = (b & 0x007FFFFFu) | 0x3F800000u; /* mantissa in [1,2) */
+  if (mb > 0x3FB504F3u) {                        /* above sqrt(2): halve */
+    mb -= 0x00800000u;
+    e += 1;
+  }
+  float f = mcfl_bitsf(mb) - 1.0f; /* exact (Sterbenz) */
+  float r = -0.07473210990428925f;
+  r = mcfl_fma(r, f, 0.128195121884346f);
+  r = mcfl_fma(r, f, -0.1326199471950531f);
+  r = mcfl_fma(r, f, 0.14196686446666718f);
+  r = mcfl_fma(r, f, -0.16608595848083496f);
+  r = mcfl_fma(r, f, 0.200011745095253f);
+  r = mcfl_fma(r, f, -0.2500157654285431f);
+  r = mcfl_fma(r, f, 0.3333333730697632f);
+  r = mcfl_fma(r, f, -0.49999988079071045f);
+  float y = mcfl_fma(f * f, r, f);
+  float ef = (float)e;
+  /* ln2 = 0.693145751953125 (16 significant bits) + 1.4286068203094172e-06 */
+  return mcfl_fma(ef, 0.693145751953125f, mcfl_fma(ef, 1.4286068203094172e-06f, y));
+}
+
+/* sin(2*pi*t) and cos(2*pi*t). */
+MCFL_HD void mcfl_sincos_turn(float t, float* sn, float* cs) {
+  float y = 4.0f * t;
+  float j = mcfl_rint(y);
+  float r = y - j; /* exact, |r| <= 1/2; angle = (pi/2)(j + r) */
+  int q = (int)j & 3;
+  float r2 = r * r;
+  float ps = mcfl_fma(0.0001582550466991961f, r2, -0.0046812924556434155f);
+  ps = mcfl_fma(ps, r2, 0.07969258725643158f);
+  ps = mcfl_fma(ps, r2, -0.6459640860557556f);
+  ps = mcfl_fma(ps, r2, 1.5707963705062866f);
+  ps = ps * r;
+  float pc = mcfl_fma(0.0009037023992277682f, r2, -0.02086012065410614f);
+  pc = mcfl_fma(pc, r2, 0.25366920232772827f);
+  pc = mcfl_fma(pc, r2, -1.2337005138397217f);
+  pc = mcfl_fma(pc, r2, 1.0f);
+  float s = (q & 1) ? pc : ps;
+  float c = (q & 1) ? ps : pc;
+  if (q & 2) s = -s;
+  if ((q + 1) & 2) c = -c;
+  *sn = s;
+  *cs = c;
+}
+
+/* ---- conversions with the saturating semantics of the GPU instructions ---- */
+
+/* uint32 -> float, rounding toward zero (keeps the 24 leading bits). */
+MCFL_HD float mcfl_u2f_rz(uint32_t u) {
+  if (u >= 0x01000000u) {
+    int sh = 8 - __builtin_clz(u);
+    u &= ~((1u << sh) - 1u);
+  }
+  return (float)u;
+}
+
+/* float -> uint32 toward zero; NaN and negatives give 0, >= 2^32 saturates. */
+MCFL_HD uint32_t mcfl_f2u_rz(float f) {
+  return (f >= 4294967296.0f) ? 0xFFFFFFFFu : ((f > 0.0f) ? (uint32_t)f : 0u);
+}
+MCFL_HD uint32_t mcfl_f2u_ru(float f) { return mcfl_f2u_rz(mcfl_ceil(f)); }
+MCFL_HD uint32_t mcfl_f2u_rn(float f) { return mcfl_f2u_rz(mcfl_rint(f)); }
+
+/* double -> uint64 toward zero with the same saturation rules. */
+MCFL_HD uint64_t mcfl_d2u64_rz(double d) {
+  return (d >= 18446744073709551616.0) ? 0xFFFFFFFFFFFFFFFFull : ((d > 0.0) ? (uint64_t)d : 0ull);
+}
+
+#endif /* MCFL_FMATH_H */

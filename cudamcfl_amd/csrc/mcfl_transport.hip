@@ -1,0 +1,543 @@
+/*
+ * mcfl_transport.hip -- persistent gfx950 kernel for CUDAMCFL's voxelized
+ * photon transport (bulk method 2).
+ *
+ * Replaces the reference's MCd3D / LaunchPhoton_Global kernels and the host
+ * relaunch loop around them (CUDAMCFLtransport.cu:242-456,460-582;
+ * CUDAMCFLmain.cu:117-165,229-273).  One launch runs a whole simulation, a
+ * batch of simulations, or the fluorescence stage of every emitting voxel:
+ *
+ *  - Work: a flattened photon index space.  Each wavefront keeps a chunk of
+ *    indices; lanes whose photon died are found with a ballot, ranked with
+ *    mbcnt, and given the next indices of the chunk; an empty chunk is
+ *    refilled with one 64-bit atomic on a global counter by the wave's first
+ *    lane (guided sizes: large early, small near the end).  No host round
+ *    trips, no relaunches.
+ *  - RNG: each photon (or each reference lane) owns an MWC stream
+ *    (mcfl_mwc.h), so results do not depend on which lane runs what.
+ *  - Medium: 8-bit labels in HBM (L2/MALL resident), per-material constants
+ *    in LDS.
+ *  - Tallies: 64-bit integer atomics (order independent); per-job scalar
+ *    totals are privatised in LDS and flushed once per workgroup.
+ *
+ * The step follows CUDAMCFLtransport.cu:289-447 quirk for quirk (end-point
+ * label lookup, z-normal Fresnel, absorption and spin at a surface point after
+ * internal reflection, sin/cos swap in Spin, integer weights, roulette).
+ */
+#include <hip/hip_runtime.h>
+
+#include "mcfl_device.h"
+#include "mcfl_mwc.h"
+
+#define MCFL_WG 256
+
+namespace {
+
+constexpr uint32_t kWeightI = 429497u;   /* CUDAMCFL.h:33 */
+constexpr float kChance = 0.1f;          /* CUDAMCFL.h:34 */
+constexpr double kCcmfs = 2.99792E-5;    /* CUDAMCFL.h:29 */
+constexpr float kPi = 3.141592654f;      /* CUDAMCFL.h:22 */
+constexpr float kFltMax = 3.402823466e+38f;
+
+struct Ph {
+  float x, y, z, dx, dy, dz;
+  uint32_t w;   /* weight, 2^-32 fixed point */
+  uint32_t lab; /* current medium label (bulkpos) */
+  uint64_t tof; /* time of flight [fs] */
+};
+
+__device__ __forceinline__ bool in_grid(const MediumDev& m, float x, float y, float z) {
+  return mcfl_fabs(x) < m.halfx && mcfl_fabs(y) < m.halfy && z < m.esp;
+}
+
+/* CUDAMCFLtransport.cu:323-325 with each component clamped into the grid. */
+__device__ __forceinline__ uint32_t vox_of(const MediumDev& m, float x, float y, float z) {
+  uint32_t ix = min(mcfl_f2u_rz((x + m.halfx) * m.gs), (uint32_t)m.nx - 1u);
+  uint32_t iy = min(mcfl_f2u_rz((y + m.halfy) * m.gs), (uint32_t)m.ny - 1u);
+  uint32_t iz = min(mcfl_f2u_rz(z * m.gs), (uint32_t)m.nz - 1u);
+  return ix + (uint32_t)m.nx * (iy + (uint32_t)m.ny * iz);
+}
+
+__device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void lds_add_u64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+/* LaunchPhoton, CUDAMCFLtransport.cu:460-562 (method 2). */
+__device__ void launch(const MediumDev& m, int kind, float xi, float yi, float zi, float src_r, uint32_t sw, Ph& p,
+                       uint64_t& x, uint32_t a) {
+  if (kind == MCFL_SRC_ISOTROPIC) {
+    const float step = 0.999999f / m.gs;
+    p.x = xi + step * mcfl_rand_oc(&x, a);
+    p.y = yi + step * mcfl_rand_oc(&x, a);
+    p.z = zi + step * mcfl_rand_oc(&x, a);
+    float ct = (float)(1.0 - 2.0 * (double)mcfl_rand_oc(&x, a));
+    float st = (float)sqrt(1.0 - (double)(ct * ct));
+    float u = mcfl_rand_oc(&x, a);
+    float psi = (float)(2.0 * (double)kPi * (double)u);
+    float sn, cp;
+    mcfl_sincos_turn(u, &sn, &cp);
+    float sp = mcfl_sqrt((float)(1.0 - (double)(cp * cp)));
+    if (!(psi < kPi)) sp = -sp;
+    p.dx = st * cp;
+    p.dy = st * sp;
+    p.dz = ct;
+    p.w = 0xFFFFFFFFu;
+  } else {
+    float u1 = mcfl_rand_oc(&x, a);
+    float u2 = mcfl_rand_oc(&x, a);
+    float rad = (kind == MCFL_SRC_PENCIL) ? 0.03f * mcfl_sqrt(-mcfl_log(u1)) : src_r * mcfl_sqrt(u1);
+    float sn, cs;
+    mcfl_sincos_turn(u2, &sn, &cs);
+    p.x = xi + rad * cs;
+    p.y = yi + rad * sn;
+    p.z = zi;
+    p.dx = 0.0f;
+    p.dy = 0.0f;
+    p.dz = 1.0f;
+    p.w = sw;
+  }
+  if (p.z < m.esp)
+    p.lab = (mcfl_fabs(p.x) < m.halfx && mcfl_fabs(p.y) < m.halfy) ? (uint32_t)m.labels[vox_of(m, p.x, p.y, p.z)] : 1u;
+  else
+    p.lab = m.last;
+  p.tof = 0;
+}
+
+/* Spin, CUDAMCFLtransport.cu:585-625. */
+__device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint32_t a) {
+  float xi = mcfl_rand_co(&x, a);
+  float t = mt.onemg2 / mcfl_fma(mt.twog, xi, mt.onemg);
+  float cost = mcfl_fma(-t, t, mt.onepg2) * mt.inv2g;
+  if (mt.twog == 0.0f) cost = 2.0f * mcfl_rand_co(&x, a) - 1.0f;
+  float sint = mcfl_sqrt(mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)));
+  float sps, cps;
+  mcfl_sincos_turn(mcfl_rand_co(&x, a), &sps, &cps);
+  const float cosp = sps, sinp = cps; /* the reference's __sincosf argument order */
+  const float dx = p.dx, dy = p.dy, dz = p.dz;
+  float tt = mcfl_sqrt(mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)));
+  float nx, ny, nz;
+  if (tt == 0.0f) {
+    nx = sint * cosp;
+    ny = sint * sinp;
+    nz = mcfl_copysign(cost, dz * cost);
+  } else {
+    float stt = sint * (1.0f / tt);
+    nx = mcfl_fma(stt, dx * dz * cosp - dy * sinp, dx * cost);
+    ny = mcfl_fma(stt, dy * dz * cosp + dx * sinp, dy * cost);
+    nz = mcfl_fma(-sint * cosp, tt, dz * cost);
+  }
+  float r = 1.0f / mcfl_sqrt(nx * nx + ny * ny + nz * nz);
+  p.dx = nx * r;
+  p.dy = ny * r;
+  p.dz = nz * r;
+}
+
+/* Reflect, CUDAMCFLtransport.cu:628-700 (bulk method 2). */
+__device__ __forceinline__ void reflect(const MatDev* mats, Ph& p, uint32_t nb, uint64_t& x, uint32_t a) {
+  const float n1 = mats[p.lab].n, n2 = mats[nb].n;
+  const float ci = mcfl_fabs(p.dz);
+  if (n1 == n2) {
+    p.lab = nb;
+    return;
+  }
+  const float ci2 = ci * ci;
+  if (n2 * n2 < n1 * n1 * (1.0f - ci2)) {
+    p.dz = -p.dz;
+    return;
+  }
+  if (ci == 1.0f) {
+    float r = (n1 - n2) / (n1 + n2);
+    if (mcfl_rand_co(&x, a) <= r * r)
+      p.dz = -p.dz;
+    else
+      p.lab = nb;
+    return;
+  }
+  float e = (n1 * n1) / (n2 * n2) * (1.0f - ci2);
+  float r = 2.0f * mcfl_sqrt(mcfl_fmax(0.0f, (1.0f - ci2) * (1.0f - e) * e * ci * ci));
+  e = e + ci2 * (1.0f - 2.0f * e);
+  r = e * ((1.0f - e - r) / ((1.0f - e + r) * (e + r)));
+  if (mcfl_rand_co(&x, a) <= r) {
+    p.dz = -p.dz;
+    return;
+  }
+  r = n1 / n2;
+  e = r * r * (1.0f - ci2);
+  p.dx *= r;
+  p.dy *= r;
+  p.dz = mcfl_copysign(mcfl_sqrt(mcfl_fmax(0.0f, 1.0f - e)), p.dz);
+  p.lab = nb;
+}
+
+/* MoveToFirstBoundary, CUDAMCFLtransport.cu:702-847: half-voxel march through
+ * a glass (mus = 0) region, then back off to the sphere of radius 0.707107/gs
+ * around the first voxel of a different label. */
+template <bool TEMPORAL>
+__device__ uint32_t glass_march(const MediumDev& m, Ph& p, uint32_t old_lab, float max_s) {
+  float step = 1.0f / (2.0f * m.gs);
+  float moved = 0.0f;
+  int64_t index = 1;
+  uint32_t present = old_lab;
+  while (present == old_lab && moved < max_s) {
+    p.x = mcfl_fma(p.dx, step, p.x);
+    p.y = mcfl_fma(p.dy, step, p.y);
+    p.z = mcfl_fma(p.dz, step, p.z);
+    if (TEMPORAL) p.tof += mcfl_d2u64_rz((double)step / kCcmfs);
+    if (mcfl_fma(p.dz, step, p.z) < 0.0f) {
+      present = 0;
+      index = -1;
+      step = -p.z / p.dz;
+    } else if (mcfl_fma(p.dz, step, p.z) > m.esp) {
+      present = m.last;
+      index = -1;
+      step = (m.esp - p.z) / p.dz;
+    } else if (mcfl_fabs(mcfl_fma(p.dx, step, p.x)) >= m.halfx || mcfl_fabs(mcfl_fma(p.dy, step, p.y)) >= m.halfy) {
+      present = 1;
+      index = -1;
+    } else {
+      index = vox_of(m, p.x, p.y, p.z);
+      present = m.labels[index];
+    }
+    moved += step;
+  }
+  if (index >= 0 && present != old_lab) {
+    const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
+    const uint32_t iz = (uint32_t)index / nxy;
+    const uint32_t rem = (uint32_t)index - iz * nxy;
+    const uint32_t iy = rem / (uint32_t)m.nx;
+    const uint32_t ix = rem - iy * (uint32_t)m.nx;
+    const double half = 1.0 / (2.0 * (double)m.gs);
+    const float cx = (float)((double)((float)ix / m.gs - m.halfx) + half);
+    const float cy = (float)((double)((float)iy / m.gs - m.halfy) + half);
+    const float cz = (float)((double)((float)iz / m.gs) + half);
+    const double rv = 0.707107 * (double)(1.0f / m.gs);
+    const float r2 = (float)(rv * rv);
+    /* roots of |p - t d - c|^2 = r2 */
+    const float qx = p.x - cx, qy = p.y - cy, qz = p.z - cz;
+    const float dd = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz;
+    const float b = p.dx * qx + p.dy * qy + p.dz * qz;
+    const float cc = (qx * qx + qy * qy + qz * qz) - r2;
+    const float sq = mcfl_sqrt(mcfl_fmax(0.0f, b * b - dd * cc));
+    const float t1 = (b - sq) / dd, t2 = (b + sq) / dd;
+    const float x1 = p.x - p.dx * t1, y1 = p.y - p.dy * t1, z1 = p.z - p.dz * t1;
+    const float x2 = p.x - p.dx * t2, y2 = p.y - p.dy * t2, z2 = p.z - p.dz * t2;
+    const float d1 = (x1 - cx) * (x1 - cx) + (y1 - cy) * (y1 - cy) + (z1 - cz) * (z1 - cz);
+    const float d2 = (x2 - cx) * (x2 - cx) + (y2 - cy) * (y2 - cy) + (z2 - cz) * (z2 - cz);
+    const bool first = d1 < d2;
+    p.x = first ? x1 : x2;
+    p.y = first ? y1 : y2;
+    p.z = first ? z1 : z2;
+    if (TEMPORAL) p.tof -= mcfl_d2u64_rz((double)(first ? t1 : t2) / kCcmfs);
+  }
+  return present;
+}
+
+/* Reflectance / transmittance exit tally, CUDAMCFLtransport.cu:369-421. */
+template <bool TEMPORAL>
+__device__ void exit_tally(const JobDev& J, const Ph& p, bool top, bool scaled, double escale,
+                           unsigned long long* st) {
+  const DetDev& d = J.det;
+  lds_add_u64(&st[top ? MCFL_TOT_R : MCFL_TOT_T], p.w);
+  if (mcfl_fabs(p.x - d.x0) < d.size_x && mcfl_fabs(p.y - d.y0) < d.size_y) {
+    const uint32_t iy = mcfl_f2u_rz((p.y - d.y0 + d.size_y) / d.dy);
+    const uint32_t ix = mcfl_f2u_rz((p.x - d.x0 + d.size_x) / d.dx);
+    const uint64_t idx = (uint64_t)iy * (uint64_t)d.nx + ix;
+    if (idx < d.npix) {
+      unsigned long long inc = scaled ? (unsigned long long)((double)p.w * escale + 0.5) : (unsigned long long)p.w;
+      atomic_add_u64((top ? J.Rd : J.Tt) + idx, inc);
+      if (J.flags & MCFL_F_SQ)
+        atomic_add_u64((top ? J.Rd2 : J.Tt2) + idx, ((unsigned long long)p.w * (unsigned long long)p.w) >> 24);
+    }
+  }
+  if (TEMPORAL && d.t_rort == (top ? 0u : 1u) && p.tof < d.t_max) {
+    for (uint32_t xp = 0; xp < d.t_ndx; ++xp)
+      for (uint32_t yp = 0; yp < d.t_ndy; ++yp) {
+        const float ddx = p.x - d.t_posx[xp], ddy = p.y - d.t_posy[yp];
+        if (ddx * ddx + ddy * ddy < d.t_r * d.t_r) {
+          const uint32_t bin = mcfl_f2u_rz(((float)p.tof / (float)d.t_max) * (float)d.t_bins);
+          if (bin < d.t_bins) atomic_add_u64(J.time + (xp + d.t_ndx * (yp + d.t_ndy * (uint64_t)bin)), p.w);
+        }
+      }
+  }
+}
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t find_job(const LaunchArgs& A, uint64_t g, uint32_t hint) {
+  const JobDev* J = A.jobs;
+  if (g >= J[hint].begin && g - J[hint].begin < J[hint].count) return hint;
+  uint32_t lo = 0, hi = A.njobs; /* last job with begin <= g */
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (J[mid].begin <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+} /* namespace */
+
+template <int WORK, bool TEMPORAL>
+__global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
+  __shared__ MatDev smat[MCFL_MAX_MATERIALS];
+  __shared__ unsigned long long stot[MCFL_MAX_JOBS * MCFL_NTOT];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nslots = (WORK == MCFL_WORK_JOBS) ? A.njobs : 1u;
+  for (uint32_t i = tid; i < A.n_mats; i += MCFL_WG) smat[i] = A.mats[i];
+  for (uint32_t i = tid; i < nslots * MCFL_NTOT; i += MCFL_WG) stot[i] = 0ull;
+  __syncthreads();
+
+  const MediumDev& m = A.med;
+  const uint32_t lane = __lane_id();
+
+  Ph p = {};
+  uint64_t x = 0;
+  uint32_t a = 0;
+  uint32_t job = 0;
+  uint32_t mbase = 0;
+  uint32_t flags = 0;
+  unsigned long long* fhd = nullptr;
+  unsigned long long* axyz = nullptr;
+  double escale = 0.0;
+  uint64_t accA = 0;
+  uint32_t iters = 0;
+  bool alive = false, retired = false;
+  uint32_t lane_item = 0;
+  uint64_t lane_left = 0;
+
+  /* wave-uniform chunk of the index space */
+  uint64_t pool_next = 0, pool_end = 0, last_base = 0;
+  bool pool_dead = false;
+
+  if (WORK == MCFL_WORK_VOXELS) {
+    mbase = A.jobs[0].mat_base;
+    flags = A.jobs[0].flags;
+    fhd = A.jobs[0].fhd;
+    axyz = A.jobs[0].axyz;
+  }
+
+  for (;;) {
+    /* ---- refill: ballot the lanes without a photon, hand out indices ---- */
+    bool need = !alive && !retired;
+    uint64_t needmask = __ballot(need);
+    while (needmask != 0ull) {
+      if (pool_next == pool_end) {
+        if (pool_dead) break;
+        uint64_t rem = A.total > last_base ? A.total - last_base : 0ull;
+        uint64_t chunk = rem / ((uint64_t)A.n_waves * 4ull);
+        chunk = chunk < 64ull ? 64ull : (chunk > A.chunk_max ? (uint64_t)A.chunk_max : chunk);
+        unsigned long long base = 0ull;
+        if (lane == 0) base = atomicAdd(A.counter, (unsigned long long)chunk);
+        base = __shfl(base, 0);
+        if (base >= A.total) {
+          pool_dead = true;
+          break;
+        }
+        last_base = base;
+        pool_next = base;
+        pool_end = (base + chunk < A.total) ? base + chunk : A.total;
+      }
+      const uint32_t n = (uint32_t)__popcll(needmask);
+      const uint64_t avail = pool_end - pool_next;
+      const uint32_t take = avail < (uint64_t)n ? (uint32_t)avail : n;
+      const uint32_t rank = lane_rank(needmask);
+      if (need && rank < take) {
+        const uint64_t g = pool_next + rank;
+        need = false;
+        alive = true;
+        accA = 0;
+        iters = 0;
+        if (WORK == MCFL_WORK_JOBS) {
+          job = find_job(A, g, job);
+          const JobDev& J = A.jobs[job];
+          const uint64_t k = g - J.begin;
+          a = A.a_tab[k & A.a_mask];
+          x = mcfl_photon_state(J.key, k, a);
+          mbase = J.mat_base;
+          flags = J.flags;
+          fhd = J.fhd;
+          axyz = J.axyz;
+          escale = J.exit_scale;
+          launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a);
+        } else if (WORK == MCFL_WORK_VOXELS) {
+          const uint64_t j = g / A.per_job;
+          const uint64_t k = g - j * A.per_job;
+          const uint32_t v = A.vox[j];
+          escale = A.vscale[j];
+          a = A.a_tab[k & A.a_mask];
+          x = mcfl_photon_state(mcfl_stream_key(A.vseed, v), k, a);
+          const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
+          const uint32_t iz = v / nxy, rem = v - iz * nxy, iy = rem / (uint32_t)m.nx, ix = rem - iy * (uint32_t)m.nx;
+          const float xi = ((float)ix / m.gs) - m.halfx;
+          const float yi = ((float)iy / m.gs) - m.halfy;
+          const float zi = ((float)iz / m.gs);
+          launch(m, MCFL_SRC_ISOTROPIC, xi, yi, zi, 0.0f, 0u, p, x, a);
+        } else { /* reference lane streams */
+          const JobDev& J = A.jobs[0];
+          lane_item = (uint32_t)g;
+          x = A.xs[lane_item];
+          a = A.as[lane_item];
+          lane_left = A.lane_q + ((uint64_t)lane_item < A.lane_rem ? 1ull : 0ull) - 1ull;
+          mbase = J.mat_base;
+          flags = J.flags;
+          fhd = J.fhd;
+          axyz = J.axyz;
+          uint64_t xl = x; /* LaunchPhoton_Global does not store the advanced state */
+          launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, xl, a);
+        }
+        unsigned long long* st = stot + job * MCFL_NTOT;
+        lds_add_u64(&st[MCFL_TOT_PHOTONS], 1ull);
+        lds_add_u64(&st[MCFL_TOT_LAUNCHED_W], p.w);
+      }
+      pool_next += take;
+      needmask = __ballot(need);
+    }
+    if (need) retired = true;
+    if (__ballot(alive) == 0ull) break;
+    if (!alive) continue;
+
+    /* ---- one transport step, CUDAMCFLtransport.cu:289-435 ---- */
+    const MatDev* mats = smat + mbase;
+    unsigned long long* st = stot + job * MCFL_NTOT;
+    const MatDev mc = mats[p.lab];
+    float s;
+    uint32_t nb;
+    bool in;
+    uint32_t vi;
+    ++iters;
+    if (mc.mutr != kFltMax) {
+      s = -mcfl_log(mcfl_rand_oc(&x, a)) * mc.mutr;
+      nb = p.lab;
+      if (mcfl_fma(s, p.dz, p.z) < 0.0f) {
+        nb = 0;
+        s = -p.z / p.dz;
+      }
+      if (mcfl_fma(s, p.dz, p.z) > m.esp) {
+        nb = m.last;
+        s = (m.esp - p.z) / p.dz;
+      }
+      p.x = mcfl_fma(p.dx, s, p.x);
+      p.y = mcfl_fma(p.dy, s, p.y);
+      p.z = mcfl_fma(p.dz, s, p.z);
+      if (TEMPORAL) p.tof += mcfl_d2u64_rz((double)(s * mc.n) / kCcmfs);
+      in = in_grid(m, p.x, p.y, p.z);
+      vi = in ? vox_of(m, p.x, p.y, p.z) : 0u;
+      if (nb != 0u && nb != m.last) nb = in ? (uint32_t)m.labels[vi] : 1u;
+    } else {
+      s = 100.0f;
+      nb = glass_march<TEMPORAL>(m, p, p.lab, s);
+      in = in_grid(m, p.x, p.y, p.z);
+      vi = in ? vox_of(m, p.x, p.y, p.z) : 0u;
+    }
+    if (nb != p.lab) reflect(mats, p, nb, x, a);
+
+    if ((flags & MCFL_F_FHD) && in) atomic_add_u64(fhd + vi, p.w);
+
+    if (p.lab == 0u || p.lab == m.last) {
+      const JobDev& J = A.jobs[job];
+      exit_tally<TEMPORAL>(J, p, p.lab == 0u, (flags & MCFL_F_SCALED) != 0u, escale, st);
+      p.w = 0u;
+      s = 0.0f;
+    }
+    if (s > 0.0f) {
+      const MatDev mm = mats[p.lab];
+      const uint32_t w = mcfl_f2u_ru(mm.mua_mutr * (float)p.w);
+      if (w > 0u) {
+        p.w -= w;
+        accA += w;
+        if ((flags & MCFL_F_AXYZ) && in) atomic_add_u64(axyz + vi, w);
+      } else {
+        lds_add_u64(&st[MCFL_TOT_ZERO_KILL], p.w);
+        p.w = 0u;
+      }
+      spin(mm, p, x, a);
+    }
+    /* PhotonSurvive, CUDAMCFLtransport.cu:850-864 */
+    bool lives;
+    if (p.w > kWeightI) {
+      lives = true;
+    } else if (p.w == 0u) {
+      lives = false;
+    } else if (mcfl_rand_co(&x, a) < kChance) {
+      const uint32_t nw = mcfl_f2u_rn((float)p.w / kChance);
+      lds_add_u64(&st[MCFL_TOT_ROUL_GAIN], (unsigned long long)(nw - p.w));
+      p.w = nw;
+      lives = true;
+    } else {
+      lds_add_u64(&st[MCFL_TOT_ROUL_LOSS], p.w);
+      p.w = 0u;
+      lives = false;
+    }
+    if (lives && iters >= MCFL_MAX_STEPS_PER_PHOTON) {
+      lds_add_u64(&st[MCFL_TOT_ABORTED], p.w);
+      p.w = 0u;
+      lives = false;
+    }
+    if (!lives) {
+      lds_add_u64(&st[MCFL_TOT_A], accA);
+      lds_add_u64(&st[MCFL_TOT_STEPS], iters);
+      alive = false;
+      if (WORK == MCFL_WORK_LANES) {
+        if (lane_left > 0ull) { /* next photon of the same lane, CUDAMCFLtransport.cu:437-440 */
+          const JobDev& J = A.jobs[0];
+          --lane_left;
+          accA = 0;
+          iters = 0;
+          launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a);
+          lds_add_u64(&st[MCFL_TOT_PHOTONS], 1ull);
+          lds_add_u64(&st[MCFL_TOT_LAUNCHED_W], p.w);
+          alive = true;
+        } else {
+          A.xs[lane_item] = x; /* CUDAMCFLmem.cu:42: advanced states go back */
+        }
+      }
+    }
+  }
+
+  __syncthreads();
+  for (uint32_t i = tid; i < nslots * MCFL_NTOT; i += MCFL_WG) {
+    const unsigned long long v = stot[i];
+    if (v != 0ull) atomic_add_u64(A.jobs[i / MCFL_NTOT].tot + (i % MCFL_NTOT), v);
+  }
+}
+
+/* ------------------------------------------------------------------ host */
+
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int* blocks_per_cu) {
+  hipError_t e;
+  const void* f;
+  if (work == MCFL_WORK_JOBS)
+    f = temporal ? (const void*)mcfl_transport_kernel<MCFL_WORK_JOBS, true>
+                 : (const void*)mcfl_transport_kernel<MCFL_WORK_JOBS, false>;
+  else if (work == MCFL_WORK_VOXELS)
+    f = temporal ? (const void*)mcfl_transport_kernel<MCFL_WORK_VOXELS, true>
+                 : (const void*)mcfl_transport_kernel<MCFL_WORK_VOXELS, false>;
+  else
+    f = temporal ? (const void*)mcfl_transport_kernel<MCFL_WORK_LANES, true>
+                 : (const void*)mcfl_transport_kernel<MCFL_WORK_LANES, false>;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, MCFL_WG, 0);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int nblocks, hipStream_t stream) {
+  dim3 grid((unsigned)nblocks), block(MCFL_WG);
+  switch (A->work * 2 + (temporal ? 1 : 0)) {
+    case MCFL_WORK_JOBS * 2 + 0: mcfl_transport_kernel<MCFL_WORK_JOBS, false><<<grid, block, 0, stream>>>(*A); break;
+    case MCFL_WORK_JOBS * 2 + 1: mcfl_transport_kernel<MCFL_WORK_JOBS, true><<<grid, block, 0, stream>>>(*A); break;
+    case MCFL_WORK_VOXELS * 2 + 0: mcfl_transport_kernel<MCFL_WORK_VOXELS, false><<<grid, block, 0, stream>>>(*A); break;
+    case MCFL_WORK_VOXELS * 2 + 1: mcfl_transport_kernel<MCFL_WORK_VOXELS, true><<<grid, block, 0, stream>>>(*A); break;
+    case MCFL_WORK_LANES * 2 + 0: mcfl_transport_kernel<MCFL_WORK_LANES, false><<<grid, block, 0, stream>>>(*A); break;
+    case MCFL_WORK_LANES * 2 + 1: mcfl_transport_kernel<MCFL_WORK_LANES, true><<<grid, block, 0, stream>>>(*A); break;
+    default: return -1;
+  }
+  return (int)hipGetLastError();
+}

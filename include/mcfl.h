@@ -1,0 +1,227 @@
+/*
+ * mcfl.h -- C ABI of the MI355X voxelized fluorescence Monte Carlo library
+ * (libmcfl.so).  Plain C types, plain pointers and sizes; no exceptions or
+ * C++ types cross it.  Every entry point names the reference interface it
+ * replaces (file:line in the CUDAMCFL repository).
+ *
+ * Units follow the reference: lengths in cm, coefficients in 1/cm, photon
+ * weights as 32-bit fixed point (0xFFFFFFFF = one photon), tallies as 64-bit
+ * sums of weights (CUDAMCFL.h:184-186).
+ *
+ * Ownership: the caller owns every host buffer it passes (RNG states,
+ * output images); the library owns device memory per context.  A context is
+ * used by one host thread at a time.  Failures are returned as mcfl_status;
+ * the library never exits the process.
+ */
+#ifndef MCFL_H
+#define MCFL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCFL_STR_LEN 500        /* CUDAMCFL.h:25 */
+#define MCFL_NUM_THREADS 32768  /* reference lanes, CUDAMCFL.h:18-20 */
+#define MCFL_NTOT 16
+
+typedef enum mcfl_status {
+  MCFL_OK = 0,
+  MCFL_ERR_ARG = 1,         /* invalid argument or inconsistent simulation */
+  MCFL_ERR_IO = 2,          /* file could not be opened / written */
+  MCFL_ERR_PARSE = 3,       /* malformed .mci input */
+  MCFL_ERR_SEED = 4,        /* invalid RNG seed (CUDAMCFLrng.cu:60-65) */
+  MCFL_ERR_PHOTONS = 5,     /* too few photons for the reference lane streams */
+  MCFL_ERR_LABEL = 6,       /* voxel label outside 0..n_bulks+1 */
+  MCFL_ERR_NOMEM = 7,       /* host or device allocation failed */
+  MCFL_ERR_DEVICE = 8,      /* HIP runtime error / no device */
+  MCFL_ERR_UNSUPPORTED = 9, /* feature outside this build (e.g. bulk method 1 on GPU) */
+  MCFL_ERR_CAPACITY = 10    /* request exceeds a stated limit */
+} mcfl_status;
+
+/* ---- reference types (CUDAMCFL.h), same fields and meaning ---- */
+
+typedef struct mcfl_layer { /* LayerStruct, CUDAMCFL.h:38-52 */
+  float z_min, z_max, mutr, mua, g, n, flc, muaf, eY, albedof;
+} mcfl_layer;
+
+typedef struct mcfl_bulk { /* BulkStruct, CUDAMCFL.h:116-127 */
+  float mutr, mua, g, n, flc, muaf, eY, albedof;
+} mcfl_bulk;
+
+typedef struct mcfl_inclusion { /* IncStruct, CUDAMCFL.h:98-114 */
+  float x, y, z, r, mutr, mua, g, n, flc, muaf, eY, albedof;
+} mcfl_inclusion;
+
+typedef struct mcfl_det { /* DetStruct, CUDAMCFL.h:71-96 */
+  float dx, dy, dz;
+  int32_t nx, ny, nz;
+  float x0, y0, sep;
+  float x0_temp_det, y0_temp_det;
+  uint32_t x_temp_numdets, y_temp_numdets;
+  float x_temp_sepdets, y_temp_sepdets, temp_det_r;
+  uint32_t temp_rort;
+  uint64_t temp_bins, max_temp;
+} mcfl_det;
+
+enum { MCFL_SOURCE_ISOTROPIC = 0, MCFL_SOURCE_PENCIL = 1, MCFL_SOURCE_DISK = 2 };
+
+typedef struct mcfl_sim { /* SimulationStruct, CUDAMCFL.h:129-170 */
+  uint64_t number_of_photons;
+  uint32_t number_of_photons_per_voxel;
+  uint32_t n_layers;
+  uint32_t n_bulks;
+  uint32_t start_weight;
+  char outp_filename[MCFL_STR_LEN];
+  char inp_filename[MCFL_STR_LEN];
+  int64_t begin, end; /* byte range of the run in the input file (echoed to _param.dat) */
+  char AorB;
+  mcfl_det det;
+  mcfl_layer* layers;
+  mcfl_bulk* bulks; /* n_bulks + 2 entries: 0 above, n_bulks+1 below */
+  mcfl_inclusion inclusion;
+  int32_t grid_size; /* voxels per cm */
+  float esp;         /* slab thickness [cm] */
+  float xi, yi, zi;  /* source position [cm] */
+  float dir;         /* 0 isotropic, 1 collimated */
+  float n_down;
+  int32_t fhd_activated;
+  int32_t do_fl_sim;
+  int32_t bulk_method;
+  uint8_t* bulk_info; /* nx*ny*nz labels, x fastest (the reference stores int16) */
+  char bulkinfo_filename[MCFL_STR_LEN];
+  int32_t do_temp_sim;
+  /* ---- extensions; the parser fills them so that reference inputs behave as before ---- */
+  int32_t nx, ny, nz; /* voxel grid: (4 esp gs)^2 x (esp gs) unless "# mcfl-dims nx ny nz" follows the bulk filename */
+  float halfx, halfy; /* lateral half extents [cm]: 2 esp, or nx/(2 gs) with the dims extension */
+  int32_t src_kind;   /* MCFL_SOURCE_*; collimated sources are pencil beams unless set to DISK */
+  float src_r;        /* disk radius [cm] */
+} mcfl_sim;
+
+/* ---- parser, RNG, writers (host only) ---- */
+
+/* read_simulation_data, CUDAMCFLio.cu:243-659.  Returns the number of runs,
+ * 0 on error.  Release with mcfl_free_simulations. */
+int mcfl_read_simulation_data(const char* filename, mcfl_sim** simulations, int ignoreAdetection);
+
+/* FreeSimulationStruct, CUDAMCFLmem.cu:219-223 (also frees bulks and labels). */
+void mcfl_free_simulations(mcfl_sim* simulations, int n_simulations);
+
+/* interpret_arg, CUDAMCFLio.cu:30-55 (-S<seed>, -A).  0 = OK, 1 = unknown argument. */
+int mcfl_interpret_arg(int argc, char** argv, uint64_t* seed, int* ignoreAdetection);
+
+/* init_RNG, CUDAMCFLrng.cu:31-90: per-lane MWC states and multipliers from the
+ * safe-primes file (NULL or "" = the library's bundled table).  0 = OK, 1 =
+ * error (missing file, invalid seed). */
+int mcfl_init_rng(uint64_t* x, uint32_t* a, uint32_t n_rng, const char* safeprimes_file, uint64_t xinit);
+
+/* Path of the bundled safeprimes_base32.txt (static string). */
+const char* mcfl_default_primes_path(void);
+
+/* Photon-stream key used by MCFL_RNG_PHOTON for stream `stream` of `seed`
+ * (no counterpart in the reference; exposed so results can be reproduced). */
+uint64_t mcfl_stream_key(uint64_t seed, uint64_t stream);
+
+/* ---- device context ---- */
+
+typedef struct mcfl_ctx mcfl_ctx;
+
+/* Opens HIP device `device`, loads the MWC multiplier table (NULL/"" = bundled). */
+mcfl_status mcfl_ctx_create(int device, const char* safeprimes_file, mcfl_ctx** out);
+void mcfl_ctx_destroy(mcfl_ctx* ctx);
+const char* mcfl_status_string(mcfl_status s);
+/* Number of compute units and resident transport workgroups per CU. */
+mcfl_status mcfl_ctx_info(mcfl_ctx* ctx, int* n_cu, int* blocks_per_cu, char* name, int name_len);
+
+/* ---- excitation ---- */
+
+enum { MCFL_RNG_PHOTON = 0, MCFL_RNG_LANES = 1 };
+
+typedef struct mcfl_rng {
+  int32_t mode;      /* MCFL_RNG_PHOTON: one MWC stream per photon, keyed by seed; MCFL_RNG_LANES: reference lanes */
+  uint64_t seed;     /* MCFL_RNG_PHOTON */
+  uint64_t* x;       /* MCFL_RNG_LANES: per-lane states (advanced in place, CUDAMCFLmem.cu:42) */
+  const uint32_t* a; /* MCFL_RNG_LANES: per-lane multipliers */
+  uint32_t n_lanes;  /* MCFL_RNG_LANES: number of lanes (32768 in the reference) */
+} mcfl_rng;
+
+enum {
+  MCFL_TOT_LAUNCHED_W = 0, /* sum of launch weights */
+  MCFL_TOT_R = 1,          /* weight leaving through the top surface (whole plane) */
+  MCFL_TOT_T = 2,          /* weight leaving through the bottom surface */
+  MCFL_TOT_A = 3,          /* absorbed weight */
+  MCFL_TOT_ROUL_GAIN = 4,  /* weight added to roulette survivors */
+  MCFL_TOT_ROUL_LOSS = 5,  /* weight of roulette victims */
+  MCFL_TOT_ZERO_KILL = 6,  /* weight dropped when the absorbed quantum rounds to 0 */
+  MCFL_TOT_PHOTONS = 7,    /* photons launched */
+  MCFL_TOT_STEPS = 8,      /* photon steps (transport loop iterations) */
+  MCFL_TOT_ABORTED = 9     /* weight of photons stopped by the per-photon step cap */
+};
+
+typedef struct mcfl_tallies {
+  uint64_t* Rd_xy;    /* det.nx*(det.ny+1), required (CUDAMCFLmem.cu:113) */
+  uint64_t* Tt_xy;    /* det.nx*(det.ny+1), required */
+  uint64_t* fhd;      /* nx*ny*nz photon hitting density; filled when fhd_activated and non-NULL */
+  uint64_t* time_xyt; /* x_temp_numdets*y_temp_numdets*temp_bins; filled when do_temp_sim and non-NULL */
+  uint64_t* A_xyz;    /* extension: absorbed weight per voxel; NULL = off */
+  uint64_t* Rd2_xy;   /* extension: per-pixel sum of (w*w)>>24 for variance; NULL = off */
+  uint64_t* Tt2_xy;
+  uint64_t totals[MCFL_NTOT];
+  uint64_t num_terminated_photons; /* photons finished (the reference's counter) */
+  double kernel_ms;                /* device time of the transport kernel (HIP events) */
+} mcfl_tallies;
+
+/* The device part of DoOneSimulation (CUDAMCFLmain.cu:60-167): runs one
+ * excitation simulation to completion in a single persistent launch and
+ * returns the integer tallies.  Outputs are overwritten. */
+mcfl_status mcfl_run_excitation(mcfl_ctx* ctx, const mcfl_sim* sim, const mcfl_rng* rng, mcfl_tallies* out);
+
+/* DoOneSimulation, CUDAMCFLmain.cu:60-193: reference lane streams (x, a of
+ * MCFL_NUM_THREADS lanes, advanced in place), writes _param/_Refl/_Trans
+ * [/_time] files, fills Fx = fhd/((2^32-1) N_finished) when fhd_activated,
+ * returns the number of photons finished (0 on error). */
+uint64_t mcfl_DoOneSimulation(mcfl_ctx* ctx, mcfl_sim* simulation, uint64_t* x, uint32_t* a, double* Fx);
+
+/* ---- fluorescence (emission) stage ---- */
+
+typedef struct mcfl_fl_result {
+  uint64_t* FlR_raw; /* optional det.nx*(det.ny+1) fixed-point tallies */
+  uint64_t* FlT_raw;
+  double unit;       /* image value of one raw unit times dx*dy */
+  uint64_t voxels;   /* emitting voxels simulated */
+  uint64_t photons;  /* emission photons simulated */
+  uint64_t totals[MCFL_NTOT];
+  double kernel_ms;
+} mcfl_fl_result;
+
+/* Emission plan of the reference's voxel loop (CUDAMCFLmain.cu:471-562):
+ * every voxel whose material has albedof < 1 emits n_per_voxel photons with
+ * weight voxelw = eY (1-albedof) Fx / (n_per_voxel (2^32-1)).  Fills vox[]
+ * (voxel indices, ascending) and scale[] (voxelw / unit); *n_vox in: capacity,
+ * out: count. */
+mcfl_status mcfl_fl_plan(const mcfl_sim* sim, const double* Fx, uint64_t n_per_voxel, uint32_t* vox, double* scale,
+                         uint64_t* n_vox, double* unit);
+
+/* The fluorescence voxel loop (CUDAMCFLmain.cu:409-647 minus file output) in
+ * one persistent launch over all emitting voxels: mua <- muaf for every
+ * material (CUDAMCFLmain.cu:458-461), isotropic sources at voxel corners,
+ * MCFL_RNG_PHOTON streams keyed by (seed, voxel).  FlR/FlT (det.nx*det.ny,
+ * may be NULL) receive the reference's images. */
+mcfl_status mcfl_run_fluorescence(mcfl_ctx* ctx, const mcfl_sim* sim, const double* Fx, uint64_t n_per_voxel,
+                                  uint64_t seed, double* FlR, double* FlT, mcfl_fl_result* res);
+
+/* ---- writers (byte layouts of the reference) ---- */
+
+/* Write_Simulation_Results, CUDAMCFLio.cu:57-191: _param.dat, _Trans.dat,
+ * _Refl.dat and (do_temp_sim) _time.dat.  0 = OK. */
+int mcfl_write_simulation_results(const mcfl_sim* sim, const mcfl_tallies* t, double simulation_time_s);
+/* _PHD-Ascii.dat, CUDAMCFLmain.cu:368-392. */
+int mcfl_write_phd(const mcfl_sim* sim, const double* Fx);
+/* _FlR.dat and _FlT.dat, CUDAMCFLmain.cu:604-647 (images det.nx*det.ny). */
+int mcfl_write_fl(const mcfl_sim* sim, const double* FlR, const double* FlT);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCFL_H */

@@ -1,0 +1,163 @@
+"""ctypes bindings of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: parity tests, smoke() and bench.py's CPU-baseline
+leg use it as the checker.  Nothing in the product imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+U64P = C.POINTER(C.c_uint64)
+U32P = C.POINTER(C.c_uint32)
+F32P = C.POINTER(C.c_float)
+F64P = C.POINTER(C.c_double)
+
+
+class Bulk(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("mutr", "mua", "g", "n", "flc", "muaf", "eY", "albedof")]
+
+
+class Sim(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("esp", C.c_float), ("gs", C.c_float), ("halfx", C.c_float), ("halfy", C.c_float),
+        ("n_bulks", C.c_uint32), ("bulks", C.POINTER(Bulk)), ("labels", C.POINTER(C.c_uint8)),
+        ("det_dx", C.c_float), ("det_dy", C.c_float), ("det_x0", C.c_float), ("det_y0", C.c_float),
+        ("det_nx", C.c_int32), ("det_ny", C.c_int32),
+        ("do_temp", C.c_int32), ("t_ndx", C.c_uint32), ("t_ndy", C.c_uint32),
+        ("t_posx", F32P), ("t_posy", F32P), ("t_r", C.c_float), ("t_rort", C.c_uint32),
+        ("t_bins", C.c_uint64), ("t_max", C.c_uint64),
+        ("src_kind", C.c_int32), ("xi", C.c_float), ("yi", C.c_float), ("zi", C.c_float), ("src_r", C.c_float),
+        ("start_weight", C.c_uint32),
+        ("fhd_on", C.c_int32), ("axyz_on", C.c_int32), ("sq_on", C.c_int32),
+    ]
+
+
+NTOT = 16
+TOT_NAMES = ["launched_w", "R", "T", "A", "roul_gain", "roul_loss", "zero_kill", "photons", "steps", "aborted"]
+
+
+class Tallies(C.Structure):
+    _fields_ = [("Rd", U64P), ("Tt", U64P), ("fhd", U64P), ("axyz", U64P), ("time", U64P),
+                ("Rd2", U64P), ("Tt2", U64P), ("tot", C.c_uint64 * NTOT)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.orc_init_rng.argtypes = [U64P, U32P, C.c_uint32, C.c_char_p, C.c_uint64]
+        L.orc_run_photons.argtypes = [C.POINTER(Sim), U32P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64,
+                                      C.c_double, C.POINTER(Tallies), C.c_int]
+        L.orc_run_lanes.argtypes = [C.POINTER(Sim), U64P, U32P, C.c_uint32, C.c_uint64, C.POINTER(Tallies), C.c_int]
+        L.orc_run_fluor_jobs.argtypes = [C.POINTER(Sim), U32P, C.c_uint32, C.c_uint32, U32P, U64P, F64P,
+                                         C.c_uint64, C.POINTER(Tallies), C.c_int]
+        L.orc_fluor_reference.restype = C.c_uint64
+        L.orc_fluor_reference.argtypes = [C.POINTER(Sim), F64P, U64P, U32P, C.c_uint32, C.c_uint64, F64P, F64P,
+                                          C.c_int]
+        L.orc_reflect_test.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, F32P, F32P]
+        L.orc_fresnel_test.restype = C.c_float
+        L.orc_fresnel_test.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.orc_spin_test.argtypes = [C.c_float, F32P, U64P, C.c_uint32, F32P]
+        L.orc_backstep_test.argtypes = [F32P, F32P, F32P, C.c_float, F32P, F32P]
+        _lib = L
+    return _lib
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class Medium:
+    """Holds a problem description (arrays kept alive) and builds orc_sim."""
+
+    def __init__(self, labels, bulks, esp, gs, halfx=None, halfy=None):
+        self.labels = np.ascontiguousarray(labels, dtype=np.uint8)  # (nz, ny, nx)
+        self.nz, self.ny, self.nx = self.labels.shape
+        self.bulks_np = np.asarray(bulks, dtype=np.float32).reshape(-1, 8)
+        self.bulks = (Bulk * len(self.bulks_np))(*[Bulk(*map(float, r)) for r in self.bulks_np])
+        self.esp = np.float32(esp)
+        self.gs = float(gs)
+        self.halfx = np.float32(2.0) * self.esp if halfx is None else np.float32(halfx)
+        self.halfy = np.float32(2.0) * self.esp if halfy is None else np.float32(halfy)
+        self.n_bulks = len(self.bulks_np) - 2
+
+    def sim(self, det=(0.1, 0.1, 0.0, 0.0, 10, 10), src=(1, 0.0, 0.0, 0.0, 0.0), start_weight=0xFFFFFFFF,
+            fhd=False, axyz=False, sq=False, temporal=None):
+        s = Sim()
+        s.nx, s.ny, s.nz = self.nx, self.ny, self.nz
+        s.esp, s.gs, s.halfx, s.halfy = float(self.esp), self.gs, float(self.halfx), float(self.halfy)
+        s.n_bulks = self.n_bulks
+        s.bulks = C.cast(self.bulks, C.POINTER(Bulk))
+        s.labels = self.labels.ctypes.data_as(C.POINTER(C.c_uint8))
+        s.det_dx, s.det_dy, s.det_x0, s.det_y0, s.det_nx, s.det_ny = det
+        s.src_kind, s.xi, s.yi, s.zi, s.src_r = src
+        s.start_weight = start_weight
+        s.fhd_on, s.axyz_on, s.sq_on = int(fhd), int(axyz), int(sq)
+        if temporal is not None:
+            posx, posy, r, rort, bins, tmax = temporal
+            self._tpx = np.asarray(posx, np.float32)
+            self._tpy = np.asarray(posy, np.float32)
+            s.do_temp, s.t_ndx, s.t_ndy = 1, len(self._tpx), len(self._tpy)
+            s.t_posx, s.t_posy = ptr(self._tpx, F32P), ptr(self._tpy, F32P)
+            s.t_r, s.t_rort, s.t_bins, s.t_max = r, rort, bins, tmax
+        return s
+
+
+class Result:
+    def __init__(self, sim, sq=False):
+        npix = sim.det_nx * (sim.det_ny + 1)
+        nvox = sim.nx * sim.ny * sim.nz
+        self.Rd = np.zeros(npix, np.uint64)
+        self.Tt = np.zeros(npix, np.uint64)
+        self.fhd = np.zeros(nvox, np.uint64) if sim.fhd_on else None
+        self.axyz = np.zeros(nvox, np.uint64) if sim.axyz_on else None
+        self.time = np.zeros(max(1, sim.t_ndx * sim.t_ndy * sim.t_bins), np.uint64) if sim.do_temp else None
+        self.Rd2 = np.zeros(npix, np.uint64) if sim.sq_on else None
+        self.Tt2 = np.zeros(npix, np.uint64) if sim.sq_on else None
+        self.t = Tallies()
+        self.t.Rd, self.t.Tt = ptr(self.Rd, U64P), ptr(self.Tt, U64P)
+        self.t.fhd, self.t.axyz, self.t.time = ptr(self.fhd, U64P), ptr(self.axyz, U64P), ptr(self.time, U64P)
+        self.t.Rd2, self.t.Tt2 = ptr(self.Rd2, U64P), ptr(self.Tt2, U64P)
+
+    @property
+    def tot(self):
+        return {n: int(self.t.tot[i]) for i, n in enumerate(TOT_NAMES)}
+
+
+def run_photons(sim, a_tab, key, k0, k1, exit_scale=0.0, nthreads=0):
+    r = Result(sim)
+    a_tab = np.ascontiguousarray(a_tab, np.uint32)
+    rc = lib().orc_run_photons(C.byref(sim), ptr(a_tab, U32P), len(a_tab), key, k0, k1, exit_scale,
+                               C.byref(r.t), nthreads)
+    assert rc == 0, rc
+    return r
+
+
+def run_lanes(sim, x, a, N, nthreads=0):
+    r = Result(sim)
+    rc = lib().orc_run_lanes(C.byref(sim), ptr(x, U64P), ptr(a, U32P), len(a), N, C.byref(r.t), nthreads)
+    if rc != 0:
+        raise ValueError(f"orc_run_lanes rc={rc}")
+    return r
+
+
+def init_rng(n, primes, seed):
+    x = np.zeros(n, np.uint64)
+    a = np.zeros(n, np.uint32)
+    rc = lib().orc_init_rng(ptr(x, U64P), ptr(a, U32P), n, primes.encode(), seed)
+    return rc, x, a
+
+
+def read_primes(path, n=None):
+    vals = np.loadtxt(path, dtype=np.uint64, usecols=0, max_rows=None if n is None else n + 1)
+    return vals.astype(np.uint32)
