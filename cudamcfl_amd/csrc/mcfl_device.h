@@ -22,16 +22,16 @@
 enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2 };
 enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2 };
 enum {
-  MCFL_TOT_LAUNCHED_W = 0,
-  MCFL_TOT_R,
-  MCFL_TOT_T,
-  MCFL_TOT_A,
-  MCFL_TOT_ROUL_GAIN,
-  MCFL_TOT_ROUL_LOSS,
-  MCFL_TOT_ZERO_KILL,
-  MCFL_TOT_PHOTONS,
-  MCFL_TOT_STEPS,
-  MCFL_TOT_ABORTED
+  MCFL_DT_LAUNCHED_W = 0,
+  MCFL_DT_R,
+  MCFL_DT_T,
+  MCFL_DT_A,
+  MCFL_DT_ROUL_GAIN,
+  MCFL_DT_ROUL_LOSS,
+  MCFL_DT_ZERO_KILL,
+  MCFL_DT_PHOTONS,
+  MCFL_DT_STEPS,
+  MCFL_DT_ABORTED
 };
 
 /* Per-material constants (32 bytes, LDS resident).  twog == 0 <=> g == 0. */

@@ -1,0 +1,612 @@
+// mcfl_host.cpp -- device contexts, RNG initialisation and the run entry
+// points of the C ABI (include/mcfl.h).
+//
+// Replaces the host halves of CUDAMCFLmain.cu (DoOneSimulation :60-193, the
+// fluorescence voxel loop :409-651), CUDAMCFLmem.cu (allocation, constant
+// upload and copy-back, :17-217) and CUDAMCFLrng.cu:31-90 (init_RNG).  Device
+// buffers live in the context and are reused across runs; one simulation is
+// one persistent kernel launch with no host polling.
+#include <dlfcn.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/mcfl.h"
+#include "mcfl_device.h"
+#include "mcfl_mwc.h"
+
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int* blocks_per_cu);
+extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int nblocks, hipStream_t stream);
+
+namespace {
+
+constexpr uint32_t kTablePhotons = 32768;  // photon-stream multipliers: lines 2..32769 of the primes file
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct mcfl_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int n_cu = 0;
+  int bpc[3][2] = {};
+  char name[256] = {};
+  uint32_t* d_atab = nullptr;
+  std::vector<uint32_t> atab;
+  DevBuf labels, mats, jobs, counter, tot, Rd, Tt, fhd, axyz, time, Rd2, Tt2, tpos, xs, as, vox, vscale;
+};
+
+namespace {
+
+mcfl_status ensure(DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.cap >= bytes) return MCFL_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) return MCFL_ERR_NOMEM;
+  b.cap = bytes;
+  return MCFL_OK;
+}
+
+void release(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+#define HIPCHK(expr)                                 \
+  do {                                               \
+    if ((expr) != hipSuccess) return MCFL_ERR_DEVICE; \
+  } while (0)
+#define STCHK(expr)                        \
+  do {                                     \
+    mcfl_status st_ = (expr);              \
+    if (st_ != MCFL_OK) return st_;        \
+  } while (0)
+
+std::string lib_dir() {
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&mcfl_default_primes_path), &info) && info.dli_fname) {
+    std::string p(info.dli_fname);
+    size_t s = p.rfind('/');
+    return s == std::string::npos ? std::string(".") : p.substr(0, s);
+  }
+  return ".";
+}
+
+// Reads the first column of n lines after the seeding line.
+bool read_multipliers(const char* path, uint32_t n, std::vector<uint32_t>& out, uint32_t* begin) {
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  unsigned a0, t1, t2;
+  if (fscanf(f, "%u %u %u", &a0, &t1, &t2) != 3) {
+    fclose(f);
+    return false;
+  }
+  if (begin) *begin = a0;
+  out.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    unsigned a;
+    if (fscanf(f, "%u %u %u", &a, &t1, &t2) != 3) {
+      fclose(f);
+      return false;
+    }
+    out[i] = a;
+  }
+  fclose(f);
+  return true;
+}
+
+// Per-material constants of the kernel (mcfl_device.h MatDev).  emission:
+// mua <- muaf (CUDAMCFLmain.cu:458-461).
+void build_mats(const mcfl_sim* s, bool emission, std::vector<MatDev>& out) {
+  const uint32_t n = s->n_bulks + 2;
+  out.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const mcfl_bulk& b = s->bulks[i];
+    const float mua = emission ? b.muaf : b.mua;
+    const float g = b.g;
+    const float g2 = g * g;
+    MatDev& m = out[i];
+    m.mutr = b.mutr;
+    m.mua_mutr = mua * b.mutr;
+    m.n = b.n;
+    m.onemg = 1.0f - g;
+    m.onemg2 = 1.0f - g2;
+    m.onepg2 = 1.0f + g2;
+    m.twog = 2.0f * g;
+    m.inv2g = (g != 0.0f) ? 1.0f / (2.0f * g) : 0.0f;
+  }
+}
+
+size_t nvox_of(const mcfl_sim* s) { return (size_t)s->nx * (size_t)s->ny * (size_t)s->nz; }
+size_t npix_of(const mcfl_sim* s) { return (size_t)s->det.nx * (size_t)(s->det.ny + 1); }
+
+mcfl_status validate(const mcfl_sim* s) {
+  if (!s) return MCFL_ERR_ARG;
+  if (s->bulk_method != 2) return MCFL_ERR_UNSUPPORTED;
+  if (s->nx <= 0 || s->ny <= 0 || s->nz <= 0 || !s->bulk_info || !s->bulks) return MCFL_ERR_ARG;
+  if (s->grid_size <= 0 || !(s->esp > 0.0f)) return MCFL_ERR_ARG;
+  if (s->n_bulks < 1 || s->n_bulks + 2 > MCFL_MAX_MATERIALS) return MCFL_ERR_CAPACITY;
+  if (s->det.nx <= 0 || s->det.ny <= 0) return MCFL_ERR_ARG;
+  if (s->do_temp_sim && (s->det.temp_bins == 0 || s->det.max_temp == 0)) return MCFL_ERR_ARG;
+  const size_t nv = nvox_of(s);
+  if (nv > 0xFFFFFFFFull) return MCFL_ERR_CAPACITY;
+  const uint8_t last = (uint8_t)(s->n_bulks + 1);
+  for (size_t i = 0; i < nv; ++i)
+    if (s->bulk_info[i] > last) return MCFL_ERR_LABEL;
+  return MCFL_OK;
+}
+
+mcfl_status medium_upload(mcfl_ctx* c, const mcfl_sim* s, MediumDev* md) {
+  const size_t nv = nvox_of(s);
+  STCHK(ensure(c->labels, nv));
+  HIPCHK(hipMemcpyAsync(c->labels.p, s->bulk_info, nv, hipMemcpyHostToDevice, c->stream));
+  md->labels = static_cast<const uint8_t*>(c->labels.p);
+  md->nx = s->nx;
+  md->ny = s->ny;
+  md->nz = s->nz;
+  md->last = s->n_bulks + 1;
+  md->esp = s->esp;
+  md->gs = (float)s->grid_size;
+  md->halfx = s->halfx;
+  md->halfy = s->halfy;
+  return MCFL_OK;
+}
+
+void fill_det(const mcfl_sim* s, DetDev* d, const float* d_tpos) {
+  d->dx = s->det.dx;
+  d->dy = s->det.dy;
+  d->x0 = s->det.x0;
+  d->y0 = s->det.y0;
+  d->size_x = (s->det.dx * (float)s->det.nx) * 0.5f;  // CUDAMCFLtransport.cu:259-260
+  d->size_y = (s->det.dy * (float)s->det.ny) * 0.5f;
+  d->nx = s->det.nx;
+  d->ny = s->det.ny;
+  d->npix = (uint32_t)npix_of(s);
+  d->t_ndx = s->det.x_temp_numdets;
+  d->t_ndy = s->det.y_temp_numdets;
+  d->t_rort = s->det.temp_rort;
+  d->t_r = s->det.temp_det_r;
+  d->t_bins = s->det.temp_bins;
+  d->t_max = s->det.max_temp;
+  d->t_posx = d_tpos;
+  d->t_posy = d_tpos ? d_tpos + s->det.x_temp_numdets : nullptr;
+}
+
+// Temporal detector positions as DoOneSimulation computes them (CUDAMCFLmain.cu:94-100).
+void tdet_positions(const mcfl_sim* s, std::vector<float>& pos) {
+  const int nx = (int)s->det.x_temp_numdets, ny = (int)s->det.y_temp_numdets;
+  pos.resize((size_t)nx + (size_t)ny);
+  for (int i = 0; i < nx; i++)
+    pos[i] = i * s->det.x_temp_sepdets - ((nx - 1) * s->det.x_temp_sepdets) / 2 + s->det.x0_temp_det;
+  for (int i = 0; i < ny; i++)
+    pos[nx + i] = i * s->det.y_temp_sepdets - ((ny - 1) * s->det.y_temp_sepdets) / 2 + s->det.y0_temp_det;
+}
+
+int src_kind_of(const mcfl_sim* s) {
+  if (s->dir == 0.0f) return MCFL_SRC_ISOTROPIC;
+  return s->src_kind == MCFL_SOURCE_DISK ? MCFL_SRC_DISK : MCFL_SRC_PENCIL;
+}
+
+mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, double* ms) {
+  const int bpc = c->bpc[A.work][temporal ? 1 : 0];
+  const int nblocks = c->n_cu * (bpc > 0 ? bpc : 1);
+  A.n_waves = (uint32_t)nblocks * 4u;
+  A.chunk_max = 1024;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (mcfl_transport_launch(&A, temporal, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  if (ms) *ms = t;
+  return MCFL_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" const char* mcfl_default_primes_path(void) {
+  static std::string path;
+  if (path.empty()) path = lib_dir() + "/../data/safeprimes_base32.txt";
+  return path.c_str();
+}
+
+extern "C" uint64_t mcfl_stream_key(uint64_t seed, uint64_t stream) { return ::mcfl_key(seed, stream); }
+
+// init_RNG, CUDAMCFLrng.cu:31-90.
+extern "C" int mcfl_init_rng(uint64_t* x, uint32_t* a, uint32_t n_rng, const char* file, uint64_t xinit) {
+  if (!file || !file[0]) file = mcfl_default_primes_path();
+  FILE* fp = fopen(file, "r");
+  if (!fp) {
+    printf("Could not find the file of safeprimes (%s)! Terminating!\n", file);
+    return 1;
+  }
+  unsigned int begin = 0u, fora = 0u, t1, t2;
+  if (fscanf(fp, "%u %u %u", &begin, &t1, &t2) != 3) {
+    fclose(fp);
+    return 1;
+  }
+  if ((xinit == 0ull) | (((unsigned int)(xinit >> 32)) >= (begin - 1)) | (((unsigned int)xinit) >= 0xfffffffful)) {
+    printf("%llu not a valid seed! Terminating!\n", (unsigned long long)xinit);
+    fclose(fp);
+    return 1;
+  }
+  for (uint32_t i = 0; i < n_rng; i++) {
+    // The reference keeps the previous multiplier when the file runs out; a
+    // short file is an error here.
+    if (fscanf(fp, "%u %u %u", &fora, &t1, &t2) != 3) {
+      fclose(fp);
+      return 1;
+    }
+    a[i] = fora;
+    uint64_t xi = 0;
+    while ((xi == 0) | (((unsigned int)(xi >> 32)) >= (fora - 1)) | (((unsigned int)xi) >= 0xfffffffful)) {
+      xinit = (xinit & 0xffffffffull) * begin + (xinit >> 32);
+      xi = (unsigned int)floor((((double)((unsigned int)xinit)) / (double)0x100000000) * fora);
+      xi <<= 32;
+      xinit = (xinit & 0xffffffffull) * begin + (xinit >> 32);
+      xi += (unsigned int)xinit;
+    }
+    x[i] = xi;
+  }
+  fclose(fp);
+  return 0;
+}
+
+extern "C" const char* mcfl_status_string(mcfl_status s) {
+  switch (s) {
+    case MCFL_OK: return "ok";
+    case MCFL_ERR_ARG: return "invalid argument";
+    case MCFL_ERR_IO: return "i/o error";
+    case MCFL_ERR_PARSE: return "malformed input";
+    case MCFL_ERR_SEED: return "invalid seed";
+    case MCFL_ERR_PHOTONS: return "too few photons for the lane streams";
+    case MCFL_ERR_LABEL: return "voxel label outside 0..n_bulks+1";
+    case MCFL_ERR_NOMEM: return "out of memory";
+    case MCFL_ERR_DEVICE: return "HIP runtime error";
+    case MCFL_ERR_UNSUPPORTED: return "unsupported";
+    case MCFL_ERR_CAPACITY: return "capacity exceeded";
+  }
+  return "unknown";
+}
+
+extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx** out) {
+  if (!out) return MCFL_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return MCFL_ERR_DEVICE;
+  mcfl_ctx* c = new (std::nothrow) mcfl_ctx();
+  if (!c) return MCFL_ERR_NOMEM;
+  c->device = device;
+  auto fail = [&](mcfl_status st) {
+    mcfl_ctx_destroy(c);
+    return st;
+  };
+  if (hipSetDevice(device) != hipSuccess) return fail(MCFL_ERR_DEVICE);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MCFL_ERR_DEVICE);
+  c->n_cu = prop.multiProcessorCount;
+  snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
+  for (int w = 0; w < 3; ++w)
+    for (int t = 0; t < 2; ++t)
+      if (mcfl_transport_occupancy(w, t, &c->bpc[w][t]) != 0) return fail(MCFL_ERR_DEVICE);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(MCFL_ERR_DEVICE);
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(MCFL_ERR_DEVICE);
+  if (!primes || !primes[0]) primes = mcfl_default_primes_path();
+  if (!read_multipliers(primes, kTablePhotons, c->atab, nullptr)) return fail(MCFL_ERR_IO);
+  if (hipMalloc(&c->d_atab, kTablePhotons * sizeof(uint32_t)) != hipSuccess) return fail(MCFL_ERR_NOMEM);
+  if (hipMemcpy(c->d_atab, c->atab.data(), kTablePhotons * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(MCFL_ERR_DEVICE);
+  *out = c;
+  return MCFL_OK;
+}
+
+extern "C" void mcfl_ctx_destroy(mcfl_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->labels, &c->mats, &c->jobs, &c->counter, &c->tot, &c->Rd, &c->Tt, &c->fhd, &c->axyz, &c->time,
+                    &c->Rd2, &c->Tt2, &c->tpos, &c->xs, &c->as, &c->vox, &c->vscale})
+    release(*b);
+  if (c->d_atab) (void)hipFree(c->d_atab);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" mcfl_status mcfl_ctx_info(mcfl_ctx* c, int* n_cu, int* bpc, char* name, int name_len) {
+  if (!c) return MCFL_ERR_ARG;
+  if (n_cu) *n_cu = c->n_cu;
+  if (bpc) *bpc = c->bpc[MCFL_WORK_JOBS][0];
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->name);
+  return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const mcfl_rng* rng, mcfl_tallies* out) {
+  if (!c || !rng || !out || !out->Rd_xy || !out->Tt_xy) return MCFL_ERR_ARG;
+  STCHK(validate(s));
+  if (s->number_of_photons == 0) return MCFL_ERR_ARG;
+  const bool lanes = rng->mode == MCFL_RNG_LANES;
+  if (lanes) {
+    if (!rng->x || !rng->a || rng->n_lanes == 0) return MCFL_ERR_ARG;
+    if (s->number_of_photons < rng->n_lanes) return MCFL_ERR_PHOTONS;  // the reference never terminates
+  } else if (rng->mode != MCFL_RNG_PHOTON) {
+    return MCFL_ERR_ARG;
+  }
+  if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
+  const int temporal = s->do_temp_sim == 1 && out->time_xyt ? 1 : 0;
+  const bool fhd_on = s->fhd_activated == 1 && out->fhd;
+  const size_t nv = nvox_of(s), npix = npix_of(s);
+  const size_t ntime = (size_t)s->det.x_temp_numdets * s->det.y_temp_numdets * s->det.temp_bins;
+
+  LaunchArgs A;
+  memset(&A, 0, sizeof A);
+  STCHK(medium_upload(c, s, &A.med));
+  std::vector<MatDev> mats;
+  build_mats(s, false, mats);
+  STCHK(ensure(c->mats, mats.size() * sizeof(MatDev)));
+  HIPCHK(hipMemcpyAsync(c->mats.p, mats.data(), mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, c->stream));
+
+  STCHK(ensure(c->Rd, npix * 8));
+  STCHK(ensure(c->Tt, npix * 8));
+  STCHK(ensure(c->tot, MCFL_MAX_JOBS * MCFL_NTOT * 8));
+  STCHK(ensure(c->counter, 8));
+  HIPCHK(hipMemsetAsync(c->Rd.p, 0, npix * 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->Tt.p, 0, npix * 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->tot.p, 0, MCFL_NTOT * 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->counter.p, 0, 8, c->stream));
+
+  JobDev J;
+  memset(&J, 0, sizeof J);
+  J.begin = 0;
+  J.count = lanes ? rng->n_lanes : s->number_of_photons;
+  J.key = ::mcfl_key(rng->seed, 0);
+  J.src_kind = src_kind_of(s);
+  J.xi = s->xi;
+  J.yi = s->yi;
+  J.zi = s->zi;
+  J.src_r = s->src_r;
+  J.start_weight = s->start_weight;
+  J.mat_base = 0;
+  J.Rd = static_cast<unsigned long long*>(c->Rd.p);
+  J.Tt = static_cast<unsigned long long*>(c->Tt.p);
+  J.tot = static_cast<unsigned long long*>(c->tot.p);
+  if (fhd_on) {
+    STCHK(ensure(c->fhd, nv * 8));
+    HIPCHK(hipMemsetAsync(c->fhd.p, 0, nv * 8, c->stream));
+    J.fhd = static_cast<unsigned long long*>(c->fhd.p);
+    J.flags |= MCFL_F_FHD;
+  }
+  if (out->A_xyz) {
+    STCHK(ensure(c->axyz, nv * 8));
+    HIPCHK(hipMemsetAsync(c->axyz.p, 0, nv * 8, c->stream));
+    J.axyz = static_cast<unsigned long long*>(c->axyz.p);
+    J.flags |= MCFL_F_AXYZ;
+  }
+  if (out->Rd2_xy && out->Tt2_xy) {
+    STCHK(ensure(c->Rd2, npix * 8));
+    STCHK(ensure(c->Tt2, npix * 8));
+    HIPCHK(hipMemsetAsync(c->Rd2.p, 0, npix * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->Tt2.p, 0, npix * 8, c->stream));
+    J.Rd2 = static_cast<unsigned long long*>(c->Rd2.p);
+    J.Tt2 = static_cast<unsigned long long*>(c->Tt2.p);
+    J.flags |= MCFL_F_SQ;
+  }
+  const float* d_tpos = nullptr;
+  if (temporal) {
+    std::vector<float> pos;
+    tdet_positions(s, pos);
+    STCHK(ensure(c->tpos, pos.size() * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(c->tpos.p, pos.data(), pos.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    STCHK(ensure(c->time, ntime * 8));
+    HIPCHK(hipMemsetAsync(c->time.p, 0, ntime * 8, c->stream));
+    J.time = static_cast<unsigned long long*>(c->time.p);
+    J.flags |= MCFL_F_TEMP;
+    d_tpos = static_cast<const float*>(c->tpos.p);
+  }
+  fill_det(s, &J.det, d_tpos);
+  STCHK(ensure(c->jobs, sizeof(JobDev)));
+  HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
+
+  A.mats = static_cast<const MatDev*>(c->mats.p);
+  A.n_mats = (uint32_t)mats.size();
+  A.counter = static_cast<unsigned long long*>(c->counter.p);
+  A.a_tab = c->d_atab;
+  A.a_mask = kTablePhotons - 1;
+  A.jobs = static_cast<const JobDev*>(c->jobs.p);
+  A.njobs = 1;
+  if (lanes) {
+    const size_t L = rng->n_lanes;
+    STCHK(ensure(c->xs, L * 8));
+    STCHK(ensure(c->as, L * 4));
+    HIPCHK(hipMemcpyAsync(c->xs.p, rng->x, L * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->as.p, rng->a, L * 4, hipMemcpyHostToDevice, c->stream));
+    A.work = MCFL_WORK_LANES;
+    A.total = L;
+    A.xs = static_cast<unsigned long long*>(c->xs.p);
+    A.as = static_cast<const uint32_t*>(c->as.p);
+    A.lane_q = s->number_of_photons / L;
+    A.lane_rem = s->number_of_photons % L;
+  } else {
+    A.work = MCFL_WORK_JOBS;
+    A.total = s->number_of_photons;
+  }
+  STCHK(launch_and_time(c, A, temporal, &out->kernel_ms));
+
+  HIPCHK(hipMemcpy(out->Rd_xy, c->Rd.p, npix * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out->Tt_xy, c->Tt.p, npix * 8, hipMemcpyDeviceToHost));
+  if (fhd_on) HIPCHK(hipMemcpy(out->fhd, c->fhd.p, nv * 8, hipMemcpyDeviceToHost));
+  if (out->A_xyz) HIPCHK(hipMemcpy(out->A_xyz, c->axyz.p, nv * 8, hipMemcpyDeviceToHost));
+  if (J.flags & MCFL_F_SQ) {
+    HIPCHK(hipMemcpy(out->Rd2_xy, c->Rd2.p, npix * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out->Tt2_xy, c->Tt2.p, npix * 8, hipMemcpyDeviceToHost));
+  }
+  if (temporal) HIPCHK(hipMemcpy(out->time_xyt, c->time.p, ntime * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out->totals, c->tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost));
+  if (lanes) HIPCHK(hipMemcpy(rng->x, c->xs.p, (size_t)rng->n_lanes * 8, hipMemcpyDeviceToHost));
+  out->num_terminated_photons = out->totals[MCFL_TOT_PHOTONS];
+  return MCFL_OK;
+}
+
+extern "C" uint64_t mcfl_DoOneSimulation(mcfl_ctx* c, mcfl_sim* s, uint64_t* x, uint32_t* a, double* Fx) {
+  const size_t nv = s ? nvox_of(s) : 0, npix = s ? npix_of(s) : 0;
+  if (!s) return 0;
+  std::vector<uint64_t> Rd(npix), Tt(npix), fhd(s->fhd_activated == 1 ? nv : 0);
+  std::vector<uint64_t> tim(s->do_temp_sim == 1 ? (size_t)s->det.x_temp_numdets * s->det.y_temp_numdets * s->det.temp_bins : 0);
+  mcfl_tallies t;
+  memset(&t, 0, sizeof t);
+  t.Rd_xy = Rd.data();
+  t.Tt_xy = Tt.data();
+  t.fhd = fhd.empty() ? nullptr : fhd.data();
+  t.time_xyt = tim.empty() ? nullptr : tim.data();
+  mcfl_rng rng;
+  memset(&rng, 0, sizeof rng);
+  rng.mode = MCFL_RNG_LANES;
+  rng.x = x;
+  rng.a = a;
+  rng.n_lanes = MCFL_NUM_THREADS;
+  clock_t t0 = clock();
+  mcfl_status st = mcfl_run_excitation(c, s, &rng, &t);
+  if (st != MCFL_OK) {
+    fprintf(stderr, "excitation failed: %s\n", mcfl_status_string(st));
+    return 0;
+  }
+  double secs = (double)(clock() - t0) / CLOCKS_PER_SEC;
+  printf("\nSimulation time: %.2f sec\n\n", secs);
+  printf("Writing excitation results...\n");
+  mcfl_write_simulation_results(s, &t, secs);
+  const uint64_t fin = t.num_terminated_photons;
+  if (Fx && !fhd.empty())
+    for (size_t v = 0; v < nv; ++v) Fx[v] = ((double)fhd[v] / (0xFFFFFFFFull * fin));  // CUDAMCFLmain.cu:181-183
+  printf("Photons simulated: %llu\n\n", (unsigned long long)fin);
+  return fin;
+}
+
+extern "C" mcfl_status mcfl_fl_plan(const mcfl_sim* s, const double* Fx, uint64_t nper, uint32_t* vox, double* scale,
+                                    uint64_t* n_vox, double* unit) {
+  if (!s || !Fx || !n_vox || !unit || nper == 0) return MCFL_ERR_ARG;
+  STCHK(validate(s));
+  const size_t nv = nvox_of(s);
+  const uint64_t cap = *n_vox;
+  uint64_t n = 0;
+  double S = 0.0;
+  for (size_t v = 0; v < nv; ++v) {
+    const mcfl_bulk& b = s->bulks[s->bulk_info[v]];
+    if (!(b.albedof < 1)) continue;  // CUDAMCFLmain.cu:551
+    // voxelw, CUDAMCFLmain.cu:553-556
+    const double w = ((double)b.eY * (double)(1 - b.albedof) * Fx[v]) / (double)(nper * 0xFFFFFFFFull);
+    if (n < cap) {
+      if (vox) vox[n] = (uint32_t)v;
+      if (scale) scale[n] = w;
+    }
+    S += w * (double)nper * 4294967296.0;
+    ++n;
+  }
+  *n_vox = n;
+  // One raw unit: the largest possible image sum maps below 2^62.
+  *unit = S > 0.0 ? S / 4611686018427387904.0 : 1.0;
+  if (scale)
+    for (uint64_t j = 0; j < n && j < cap; ++j) scale[j] /= *unit;
+  return n <= cap ? MCFL_OK : MCFL_ERR_CAPACITY;
+}
+
+extern "C" mcfl_status mcfl_run_fluorescence(mcfl_ctx* c, const mcfl_sim* s, const double* Fx, uint64_t nper,
+                                             uint64_t seed, double* FlR, double* FlT, mcfl_fl_result* res) {
+  if (!c || !s || !Fx || nper == 0) return MCFL_ERR_ARG;
+  STCHK(validate(s));
+  if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
+  const size_t nv = nvox_of(s), npix = npix_of(s);
+  std::vector<uint32_t> vox(nv);
+  std::vector<double> scale(nv);
+  uint64_t nvx = nv;
+  double unit = 1.0;
+  STCHK(mcfl_fl_plan(s, Fx, nper, vox.data(), scale.data(), &nvx, &unit));
+
+  LaunchArgs A;
+  memset(&A, 0, sizeof A);
+  STCHK(medium_upload(c, s, &A.med));
+  std::vector<MatDev> mats;
+  build_mats(s, true, mats);
+  STCHK(ensure(c->mats, mats.size() * sizeof(MatDev)));
+  HIPCHK(hipMemcpyAsync(c->mats.p, mats.data(), mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, c->stream));
+  STCHK(ensure(c->Rd, npix * 8));
+  STCHK(ensure(c->Tt, npix * 8));
+  STCHK(ensure(c->tot, MCFL_MAX_JOBS * MCFL_NTOT * 8));
+  STCHK(ensure(c->counter, 8));
+  STCHK(ensure(c->vox, (nvx ? nvx : 1) * 4));
+  STCHK(ensure(c->vscale, (nvx ? nvx : 1) * 8));
+  HIPCHK(hipMemsetAsync(c->Rd.p, 0, npix * 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->Tt.p, 0, npix * 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->tot.p, 0, MCFL_NTOT * 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->counter.p, 0, 8, c->stream));
+  if (nvx) {
+    HIPCHK(hipMemcpyAsync(c->vox.p, vox.data(), nvx * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->vscale.p, scale.data(), nvx * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  JobDev J;
+  memset(&J, 0, sizeof J);
+  J.count = nvx * nper;
+  J.src_kind = MCFL_SRC_ISOTROPIC;
+  J.start_weight = 0xFFFFFFFFu;
+  J.flags = MCFL_F_SCALED;
+  J.Rd = static_cast<unsigned long long*>(c->Rd.p);
+  J.Tt = static_cast<unsigned long long*>(c->Tt.p);
+  J.tot = static_cast<unsigned long long*>(c->tot.p);
+  fill_det(s, &J.det, nullptr);
+  STCHK(ensure(c->jobs, sizeof(JobDev)));
+  HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
+
+  A.mats = static_cast<const MatDev*>(c->mats.p);
+  A.n_mats = (uint32_t)mats.size();
+  A.work = MCFL_WORK_VOXELS;
+  A.total = nvx * nper;
+  A.counter = static_cast<unsigned long long*>(c->counter.p);
+  A.a_tab = c->d_atab;
+  A.a_mask = kTablePhotons - 1;
+  A.jobs = static_cast<const JobDev*>(c->jobs.p);
+  A.njobs = 1;
+  A.vox = static_cast<const uint32_t*>(c->vox.p);
+  A.vscale = static_cast<const double*>(c->vscale.p);
+  A.per_job = nper;
+  A.vseed = seed;
+  double ms = 0.0;
+  if (A.total > 0) STCHK(launch_and_time(c, A, 0, &ms));
+
+  std::vector<uint64_t> R(npix), T(npix);
+  HIPCHK(hipMemcpy(R.data(), c->Rd.p, npix * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(T.data(), c->Tt.p, npix * 8, hipMemcpyDeviceToHost));
+  const double dxdy = (double)s->det.dx * (double)s->det.dy;
+  const size_t xy = (size_t)s->det.nx * (size_t)s->det.ny;
+  for (size_t i = 0; i < xy; ++i) {
+    if (FlR) FlR[i] = (double)R[i] * unit / dxdy;
+    if (FlT) FlT[i] = (double)T[i] * unit / dxdy;
+  }
+  if (res) {
+    if (res->FlR_raw) memcpy(res->FlR_raw, R.data(), npix * 8);
+    if (res->FlT_raw) memcpy(res->FlT_raw, T.data(), npix * 8);
+    res->unit = unit;
+    res->voxels = nvx;
+    res->photons = nvx * nper;
+    res->kernel_ms = ms;
+    HIPCHK(hipMemcpy(res->totals, c->tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost));
+  }
+  return MCFL_OK;
+}

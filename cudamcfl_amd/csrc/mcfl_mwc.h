@@ -46,7 +46,7 @@ MCFL_HD uint64_t mcfl_mix64(uint64_t z) {
 #define MCFL_GOLDEN64 0x9E3779B97F4A7C15ull
 
 /* Key of stream number `stream` under user seed `seed` (job keys, voxel keys). */
-MCFL_HD uint64_t mcfl_stream_key(uint64_t seed, uint64_t stream) {
+MCFL_HD uint64_t mcfl_key(uint64_t seed, uint64_t stream) {
   return mcfl_mix64(mcfl_mix64(seed ^ 0xD1B54A32D192ED03ull) + (stream + 1ull) * MCFL_GOLDEN64);
 }
 

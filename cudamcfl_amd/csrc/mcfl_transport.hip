@@ -241,7 +241,7 @@ template <bool TEMPORAL>
 __device__ void exit_tally(const JobDev& J, const Ph& p, bool top, bool scaled, double escale,
                            unsigned long long* st) {
   const DetDev& d = J.det;
-  lds_add_u64(&st[top ? MCFL_TOT_R : MCFL_TOT_T], p.w);
+  lds_add_u64(&st[top ? MCFL_DT_R : MCFL_DT_T], p.w);
   if (mcfl_fabs(p.x - d.x0) < d.size_x && mcfl_fabs(p.y - d.y0) < d.size_y) {
     const uint32_t iy = mcfl_f2u_rz((p.y - d.y0 + d.size_y) / d.dy);
     const uint32_t ix = mcfl_f2u_rz((p.x - d.x0 + d.size_x) / d.dx);
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
           const uint32_t v = A.vox[j];
           escale = A.vscale[j];
           a = A.a_tab[k & A.a_mask];
-          x = mcfl_photon_state(mcfl_stream_key(A.vseed, v), k, a);
+          x = mcfl_photon_state(mcfl_key(A.vseed, v), k, a);
           const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
           const uint32_t iz = v / nxy, rem = v - iz * nxy, iy = rem / (uint32_t)m.nx, ix = rem - iy * (uint32_t)m.nx;
           const float xi = ((float)ix / m.gs) - m.halfx;
@@ -395,8 +395,8 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
           launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, xl, a);
         }
         unsigned long long* st = stot + job * MCFL_NTOT;
-        lds_add_u64(&st[MCFL_TOT_PHOTONS], 1ull);
-        lds_add_u64(&st[MCFL_TOT_LAUNCHED_W], p.w);
+        lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
+        lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
       }
       pool_next += take;
       needmask = __ballot(need);
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
         accA += w;
         if ((flags & MCFL_F_AXYZ) && in) atomic_add_u64(axyz + vi, w);
       } else {
-        lds_add_u64(&st[MCFL_TOT_ZERO_KILL], p.w);
+        lds_add_u64(&st[MCFL_DT_ZERO_KILL], p.w);
         p.w = 0u;
       }
       spin(mm, p, x, a);
@@ -469,22 +469,22 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
       lives = false;
     } else if (mcfl_rand_co(&x, a) < kChance) {
       const uint32_t nw = mcfl_f2u_rn((float)p.w / kChance);
-      lds_add_u64(&st[MCFL_TOT_ROUL_GAIN], (unsigned long long)(nw - p.w));
+      lds_add_u64(&st[MCFL_DT_ROUL_GAIN], (unsigned long long)(nw - p.w));
       p.w = nw;
       lives = true;
     } else {
-      lds_add_u64(&st[MCFL_TOT_ROUL_LOSS], p.w);
+      lds_add_u64(&st[MCFL_DT_ROUL_LOSS], p.w);
       p.w = 0u;
       lives = false;
     }
     if (lives && iters >= MCFL_MAX_STEPS_PER_PHOTON) {
-      lds_add_u64(&st[MCFL_TOT_ABORTED], p.w);
+      lds_add_u64(&st[MCFL_DT_ABORTED], p.w);
       p.w = 0u;
       lives = false;
     }
     if (!lives) {
-      lds_add_u64(&st[MCFL_TOT_A], accA);
-      lds_add_u64(&st[MCFL_TOT_STEPS], iters);
+      lds_add_u64(&st[MCFL_DT_A], accA);
+      lds_add_u64(&st[MCFL_DT_STEPS], iters);
       alive = false;
       if (WORK == MCFL_WORK_LANES) {
         if (lane_left > 0ull) { /* next photon of the same lane, CUDAMCFLtransport.cu:437-440 */
@@ -493,8 +493,8 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
           accA = 0;
           iters = 0;
           launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a);
-          lds_add_u64(&st[MCFL_TOT_PHOTONS], 1ull);
-          lds_add_u64(&st[MCFL_TOT_LAUNCHED_W], p.w);
+          lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
+          lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
           alive = true;
         } else {
           A.xs[lane_item] = x; /* CUDAMCFLmem.cu:42: advanced states go back */

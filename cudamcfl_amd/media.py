@@ -1,0 +1,121 @@
+"""Seeded voxel-medium generator and bulk-descriptor / .mci writers.
+
+Shapes follow the reference's medium notebook (spheres and axis-parallel
+cylinders, Bulk_descriptor_creator.ipynb:30-185); the bulk file is written in
+its on-disk order, x fastest, one integer label per voxel separated by spaces
+(Bulk_descriptor_creator.ipynb:249-254, read back by CUDAMCFLio.cu:618-636).
+Optical ranges are the benchmark's, converted from 1/mm to the reference's
+1/cm (mu[1/cm] = 10 mu[1/mm]).
+"""
+import numpy as np
+
+from . import Simulation, SOURCE_PENCIL
+
+# rows: n, mua, mus, flc, AbsCExi, AbsCEmi, eY, g (1/cm), .mci medium-line order
+
+
+def random_materials(rng, n_mat=8):
+    """Material table: mua U[0.05,1.0]/cm, mus U[50,200]/cm, g U[0.8,0.95],
+    n = 1.4, fluorophore absorption U[0,0.2]/cm (flc = 1, AbsCEmi = AbsCExi),
+    quantum yield 0.1."""
+    rows = []
+    for _ in range(n_mat):
+        mua = rng.uniform(0.05, 1.0)
+        mus = rng.uniform(50.0, 200.0)
+        g = rng.uniform(0.8, 0.95)
+        fl = rng.uniform(0.0, 0.2)
+        rows.append([1.4, mua, mus, 1.0, fl, fl, 0.1, g])
+    return np.asarray(rows, np.float32)
+
+
+def inclusions(shape, rng, n_spheres=5, r_sphere=(3, 8), n_cyl=2, r_cyl=(2, 5), first_label=2):
+    """uint8 labels (nz, ny, nx): background 1, then one new label per
+    inclusion (spheres, then z-parallel cylinders), centres uniform inside."""
+    nz, ny, nx = shape
+    lab = np.ones(shape, np.uint8)
+    z, y, x = np.ogrid[0:nz, 0:ny, 0:nx]
+    label = first_label
+    for _ in range(n_spheres):
+        r = min(rng.uniform(*r_sphere), 0.5 * min(shape))
+        c = rng.uniform([r, r, r], [nz - r, ny - r, nx - r])
+        lab[(z + 0.5 - c[0]) ** 2 + (y + 0.5 - c[1]) ** 2 + (x + 0.5 - c[2]) ** 2 < r * r] = label
+        label += 1
+    for _ in range(n_cyl):
+        r = min(rng.uniform(*r_cyl), 0.5 * min(ny, nx))
+        c = rng.uniform([r, r], [ny - r, nx - r])
+        mask = ((y + 0.5 - c[0]) ** 2 + (x + 0.5 - c[1]) ** 2 < r * r)
+        lab[np.broadcast_to(mask, shape)] = label
+        label += 1
+    return lab
+
+
+def cfg2_medium(seed=2024, shape=(64, 128, 128)):
+    """128x128x64 medium of 0.5 mm voxels: 5 spheres r in [3,8] voxels, 2
+    z-parallel cylinders r in [2,5] voxels, 8 materials."""
+    rng = np.random.default_rng(seed)
+    lab = inclusions(shape, rng)
+    mats = random_materials(rng, 8)
+    return lab, mats
+
+
+def cfg2_simulation(photons=100_000_000, seed=2024, fhd=True, source=(0.0, 0.0, 0.0)):
+    """The plain-path benchmark problem: grid_size 20 /cm (0.5 mm voxels),
+    slab 3.2 cm, 128x128 whole-field images of 0.5 mm pixels, pencil beam."""
+    lab, mats = cfg2_medium(seed)
+    nz, ny, nx = lab.shape
+    gs = 20
+    return Simulation.voxel(lab, mats, esp=nz / gs, grid_size=gs, det=(1.0 / gs, 1.0 / gs, nx, ny, 0.0, 0.0),
+                            photons=photons, source=source, src_kind=SOURCE_PENCIL, n_above=1.0, n_below=1.0,
+                            fhd=fhd)
+
+
+def write_bulk(path, labels):
+    """Bulk-descriptor file: labels x fastest, '%i' separated by spaces, one line."""
+    flat = np.asarray(labels, np.uint8).reshape(-1)  # (nz, ny, nx) C-order == x fastest
+    with open(path, "w") as f:
+        f.write(" ".join(map(str, flat.tolist())))
+        f.write("\n")
+
+
+def write_mci(path, out_name, bulk_file, mats, esp, grid_size, photons, photons_per_voxel=40000, do_fl=1,
+              det=(0.05, 0.05, 128, 128, 0.0, 0.0), source=(0.0, 0.0, 0.0), n_above=1.0, n_below=1.0, dims=None,
+              temporal=None):
+    """A method-2 .mci file in the reference grammar (CUDAMCFLio.cu:243-659).
+    dims=(nx,ny,nz) appends the '# mcfl-dims' extension line after the bulk
+    filename (a comment to the reference parser)."""
+    dx, dy, nx, ny, x0, y0 = det
+    t = temporal or (0.0, 0.0, 1, 1, 1.0, 1.0, 0.5, 0, 10, 1000000)
+    lines = [
+        "1.0 # file version",
+        "1 # number of runs",
+        "2 # bulk description method",
+        f"{out_name} A # output file name, ASCII",
+        f"{int(photons)} # No. of photons",
+        f"{int(photons_per_voxel)} # No. of photons per voxel (fluorescence)",
+        f"{int(grid_size)} # Grid finesse",
+        f"{int(do_fl)} # Do fluorescence simulation?",
+        f"{dx} {dy} # Image dx, dy",
+        f"{nx} {ny} # Image No. of dx, dy",
+        f"{x0} {y0} # Image x0, y0",
+        f"{source[0]} {source[1]} {source[2]} # Source position x y z",
+        f"{1 if temporal else 0} # Do temporal simulation?",
+        f"{t[0]} {t[1]} # Detector array center x,y",
+        f"{t[2]} {t[3]} # Number of detectors x,y",
+        f"{t[4]} {t[5]} # Separation between detectors",
+        f"{t[6]} # Temporal detector radius",
+        f"{t[7]} # Location of the detector array",
+        f"{t[8]} # Number of temporal bins",
+        f"{t[9]} # Max temp [fs]",
+        "",
+        f"{len(mats)} # Number of bulk descriptors",
+        f"{esp} # Bulk thickness",
+        f"{n_above} # n for medium above",
+    ]
+    for r in mats:
+        lines.append(" ".join(repr(float(v)) for v in r))
+    lines.append(f"{n_below} # n for medium below")
+    lines.append(bulk_file)
+    if dims is not None:
+        lines.append(f"# mcfl-dims {dims[0]} {dims[1]} {dims[2]}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

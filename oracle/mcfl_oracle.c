@@ -708,3 +708,27 @@ void orc_spin_test(float g, const float* din, uint64_t* x, uint32_t a, float* do
 void orc_backstep_test(const float* p, const float* d, const float* cc, float r2, float* t1, float* t2) {
   backstep(p[0], p[1], p[2], d[0], d[1], d[2], cc[0], cc[1], cc[2], r2, t1, t2);
 }
+
+/* ----------------------------------------------- math / generator exposure */
+
+void orc_math_eval(int fn, const float* in, float* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    float s, c;
+    switch (fn) {
+      case 0: out[i] = mcfl_log(in[i]); break;
+      case 1: mcfl_sincos_turn(in[i], &s, &c); out[i] = s; break;
+      case 2: mcfl_sincos_turn(in[i], &s, &c); out[i] = c; break;
+      case 3: { uint32_t u; memcpy(&u, &in[i], 4); out[i] = mcfl_u2f_rz(u); } break;
+      default: out[i] = 0.0f;
+    }
+  }
+}
+
+void orc_mwc_draws(uint64_t x, uint32_t a, size_t n, uint64_t* states, float* co) {
+  for (size_t i = 0; i < n; ++i) {
+    co[i] = mcfl_rand_co(&x, a);
+    states[i] = x;
+  }
+}
+
+uint64_t orc_photon_state(uint64_t key, uint64_t k, uint32_t a) { return mcfl_photon_state(key, k, a); }

@@ -161,3 +161,91 @@ def init_rng(n, primes, seed):
 def read_primes(path, n=None):
     vals = np.loadtxt(path, dtype=np.uint64, usecols=0, max_rows=None if n is None else n + 1)
     return vals.astype(np.uint32)
+
+
+def tdet_positions(n, sep, x0):
+    """Temporal detector positions in float32, same operation order as the
+    reference's DoOneSimulation (CUDAMCFLmain.cu:95-100)."""
+    f = np.float32
+    return np.array([f(f(f(i) * f(sep)) - f(f(f(n - 1) * f(sep)) / f(2))) + f(x0) for i in range(n)], np.float32)
+
+
+class FromProduct:
+    """orc_sim built from a cudamcfl_amd.Simulation (same arrays, same numbers)."""
+
+    def __init__(self, sim, emission=False, axyz=False, sq=False):
+        s = sim.s
+        nb = s.n_bulks + 2
+        rows = []
+        for i in range(nb):
+            b = s.bulks[i]
+            mua = b.muaf if emission else b.mua
+            rows.append([b.mutr, mua, b.g, b.n, b.flc, b.muaf, b.eY, b.albedof])
+        self.bulks = (Bulk * nb)(*[Bulk(*map(float, r)) for r in np.asarray(rows, np.float32)])
+        self.labels = np.array(sim.labels, copy=True)
+        o = Sim()
+        o.nx, o.ny, o.nz = s.nx, s.ny, s.nz
+        o.esp, o.gs, o.halfx, o.halfy = s.esp, float(s.grid_size), s.halfx, s.halfy
+        o.n_bulks = s.n_bulks
+        o.bulks = C.cast(self.bulks, C.POINTER(Bulk))
+        o.labels = self.labels.ctypes.data_as(C.POINTER(C.c_uint8))
+        d = s.det
+        o.det_dx, o.det_dy, o.det_x0, o.det_y0, o.det_nx, o.det_ny = d.dx, d.dy, d.x0, d.y0, d.nx, d.ny
+        if s.do_temp_sim == 1 and not emission:
+            self._tpx = tdet_positions(d.x_temp_numdets, d.x_temp_sepdets, d.x0_temp_det)
+            self._tpy = tdet_positions(d.y_temp_numdets, d.y_temp_sepdets, d.y0_temp_det)
+            o.do_temp, o.t_ndx, o.t_ndy = 1, d.x_temp_numdets, d.y_temp_numdets
+            o.t_posx, o.t_posy = ptr(self._tpx, F32P), ptr(self._tpy, F32P)
+            o.t_r, o.t_rort, o.t_bins, o.t_max = d.temp_det_r, d.temp_rort, d.temp_bins, d.max_temp
+        o.src_kind = 0 if s.dir == 0.0 else (2 if s.src_kind == 2 else 1)
+        o.xi, o.yi, o.zi, o.src_r = s.xi, s.yi, s.zi, s.src_r
+        o.start_weight = s.start_weight
+        o.fhd_on = 1 if (s.fhd_activated == 1 and not emission) else 0
+        o.axyz_on, o.sq_on = int(axyz), int(sq)
+        self.o = o
+
+
+def run_fluor_jobs(osim, a_tab, vox, keys, scale, n_per_job, nthreads=0):
+    r = Result(osim)
+    vox = np.ascontiguousarray(vox, np.uint32)
+    keys = np.ascontiguousarray(keys, np.uint64)
+    scale = np.ascontiguousarray(scale, np.float64)
+    a_tab = np.ascontiguousarray(a_tab, np.uint32)
+    rc = lib().orc_run_fluor_jobs(C.byref(osim), ptr(a_tab, U32P), len(a_tab), len(vox), ptr(vox, U32P),
+                                  ptr(keys, U64P), ptr(scale, F64P), n_per_job, C.byref(r.t), nthreads)
+    assert rc == 0, rc
+    return r
+
+
+def _bind_extras(L):
+    if getattr(L, "_extras", False):
+        return
+    L.orc_math_eval.argtypes = [C.c_int, F32P, F32P, C.c_size_t]
+    L.orc_mwc_draws.argtypes = [C.c_uint64, C.c_uint32, C.c_size_t, U64P, F32P]
+    L.orc_photon_state.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
+    L.orc_photon_state.restype = C.c_uint64
+    L._extras = True
+
+
+def math_eval(fn, x):
+    L = lib()
+    _bind_extras(L)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    L.orc_math_eval(fn, ptr(x, F32P), ptr(out, F32P), len(x))
+    return out
+
+
+def mwc_draws(x, a, n):
+    L = lib()
+    _bind_extras(L)
+    st = np.zeros(n, np.uint64)
+    co = np.zeros(n, np.float32)
+    L.orc_mwc_draws(x, a, n, ptr(st, U64P), ptr(co, F32P))
+    return st, co
+
+
+def photon_state(key, k, a):
+    L = lib()
+    _bind_extras(L)
+    return int(L.orc_photon_state(key, k, a))

@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libmcfl.so on the GPU)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import cudamcfl_amd as M
+    c = M.Context(0)
+    yield c
+    c.close()

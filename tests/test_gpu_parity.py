@@ -1,0 +1,107 @@
+"""GPU transport vs the CPU oracle on the same inputs: integer tallies must be
+bit-identical (the kernel and the oracle execute the same FP32 arithmetic)."""
+import numpy as np
+import pytest
+
+import cudamcfl_amd as M
+import _oracle as O
+from media import small_sim, small_labels, MATS3
+
+pytestmark = pytest.mark.gpu
+
+A_TAB = None
+
+
+def a_tab():
+    global A_TAB
+    if A_TAB is None:
+        A_TAB = M.multipliers(32768)
+    return A_TAB
+
+
+def assert_same(g, o, names=("Rd", "Tt", "fhd", "A_xyz", "time", "Rd2", "Tt2")):
+    omap = {"A_xyz": "axyz"}
+    for n in names:
+        ga = getattr(g, n, None)
+        oa = getattr(o, omap.get(n, n), None)
+        if ga is None or oa is None:
+            continue
+        k = min(len(ga), len(oa))
+        diff = np.nonzero(ga[:k] != oa[:k])[0]
+        assert len(diff) == 0, f"{n}: {len(diff)} entries differ, first {diff[:5]} gpu {ga[diff[:5]]} cpu {oa[diff[:5]]}"
+
+
+def check_totals(gt, ot):
+    for k in O.TOT_NAMES:
+        assert gt[k] == ot[k], f"total {k}: gpu {gt[k]} cpu {ot[k]}"
+
+
+def balance(t):
+    return t["launched_w"] + t["roul_gain"] - t["roul_loss"] - t["zero_kill"] - t["aborted"] - (t["R"] + t["T"] + t["A"])
+
+
+def test_photon_streams_bit_exact(ctx):
+    sim = small_sim(photons=20000)
+    print("case: photon streams, 20x20x10, 20000 photons, seed 7", flush=True)
+    g = ctx.run_excitation(sim, seed=7, axyz=True, sq=True)
+    o = O.FromProduct(sim, axyz=True, sq=True)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(7, 0), 0, 20000)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert g.totals["photons"] == 20000
+    assert balance(g.totals) == 0
+    assert g.Rd.sum() > 0 and g.Tt.sum() > 0 and g.time.sum() > 0
+
+
+def test_lane_streams_bit_exact(ctx):
+    sim = small_sim(photons=5003, temporal=False)
+    x, a = M.init_rng(512, seed=12345)
+    x_cpu = x.copy()
+    print("case: reference lane streams, 512 lanes, 5003 photons", flush=True)
+    g = ctx.run_excitation(sim, lanes=(x, a))
+    o = O.FromProduct(sim)
+    r = O.run_lanes(o.o, x_cpu, a, 5003)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert np.array_equal(x, x_cpu), "advanced lane states differ"
+    assert g.totals["photons"] == 5003
+
+
+def test_lanes_need_enough_photons(ctx):
+    sim = small_sim(photons=100, temporal=False)
+    x, a = M.init_rng(512, seed=3)
+    with pytest.raises(M.McflError, match="too few photons"):
+        ctx.run_excitation(sim, lanes=(x, a))
+
+
+def test_fluorescence_bit_exact(ctx):
+    lab = small_labels(10, 10, 5)
+    sim = M.Simulation.voxel(lab, MATS3, esp=0.5, grid_size=10, det=(0.1, 0.1, 10, 10, 0.0, 0.0), photons=20000,
+                             n_above=1.0, n_below=1.33, fhd=True)
+    print("case: fluorescence stage, 10x10x5, 40 photons/voxel", flush=True)
+    ex = ctx.run_excitation(sim, seed=11)
+    Fx = ex.fhd.astype(np.float64) / (0xFFFFFFFF * ex.totals["photons"])
+    fl = ctx.run_fluorescence(sim, Fx, 40, seed=99)
+    vox, scale, unit = M.fl_plan(sim, Fx, 40)
+    assert fl["voxels"] == len(vox) and fl["unit"] == unit
+    keys = np.array([M.stream_key(99, int(v)) for v in vox], np.uint64)
+    o = O.FromProduct(sim, emission=True)
+    r = O.run_fluor_jobs(o.o, a_tab(), vox, keys, scale, 40)
+    assert np.array_equal(fl["raw_R"], r.Rd) and np.array_equal(fl["raw_T"], r.Tt)
+    check_totals(fl["totals"], r.tot)
+    assert fl["raw_R"].sum() > 0
+    np.testing.assert_allclose(fl["FlR"], r.Rd[:100] * unit / (0.1 * 0.1), rtol=1e-12)
+
+
+def test_glass_inclusion_bit_exact(ctx):
+    lab = small_labels()
+    lab[2:5, 5:12, 5:12] = 4  # a glass (mus = 0) block
+    mats = MATS3 + [[1.45, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.9]]
+    sim = M.Simulation.voxel(lab, mats, esp=1.0, grid_size=10, det=(0.1, 0.1, 20, 20, 0.0, 0.0), photons=8000,
+                             n_above=1.0, n_below=1.0, fhd=True)
+    print("case: glass inclusion, 8000 photons", flush=True)
+    g = ctx.run_excitation(sim, seed=5, axyz=True)
+    o = O.FromProduct(sim, axyz=True)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(5, 0), 0, 8000)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)

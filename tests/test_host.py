@@ -1,0 +1,217 @@
+"""Host-side C ABI checks that need no GPU: exported symbols, the .mci parser
+against hand-derived values of the reference's own sample inputs, writer byte
+layouts, the emission plan, and the command-line front end."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cudamcfl_amd as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_INPUTS = os.path.join(ROOT, "tests", "golden", "ref_inputs")
+f32 = np.float32
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "mcfl.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(mcfl_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_function():
+    names = header_functions()
+    assert len(names) >= 17
+    lib = M.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(M.EXPORTS) | set(names)
+
+
+@pytest.fixture
+def in_ref_inputs(monkeypatch):
+    monkeypatch.chdir(REF_INPUTS)
+
+
+def test_parse_sample2(in_ref_inputs):
+    sim = M.Simulation.from_mci("sample2.mci")
+    s = sim.s
+    assert sim.n_runs == 1 and s.bulk_method == 2
+    assert s.outp_filename == b"sample2-out" and s.AorB == b"A"
+    assert s.number_of_photons == 500000000 and s.number_of_photons_per_voxel == 40000
+    assert s.grid_size == 5 and s.do_fl_sim == 1 and s.fhd_activated == 1
+    assert (s.det.dx, s.det.dy) == (f32(0.04), f32(0.04)) and (s.det.nx, s.det.ny) == (250, 250)
+    assert (s.xi, s.yi, s.zi) == (-4.0, 0.0, 0.0)
+    assert s.do_temp_sim == 1 and (s.det.x_temp_numdets, s.det.y_temp_numdets) == (2, 1)
+    assert s.det.temp_det_r == f32(0.5) and s.det.temp_rort == 0
+    assert s.det.temp_bins == 1000 and s.det.max_temp == 20000000
+    assert s.n_bulks == 2 and s.esp == 5.0
+    # grid (4 esp gs)^2 x (esp gs) = 100 x 100 x 25 = 250000 = digits in 3dbulk-sample
+    assert (s.nx, s.ny, s.nz) == (100, 100, 25) and s.halfx == 10.0
+    b = [s.bulks[i] for i in range(4)]
+    assert b[0].n == 1.0 and b[0].mutr == np.finfo(f32).max and b[0].g == 1.0
+    assert b[3].n == 1.0 and b[3].mua == 0.0
+    # medium 1: mua + flc*AbsCExi = 0.04 + 1e-8*339625 = 0.04339625; muaf = 0.04 + 1e-8*219500
+    assert abs(b[1].mua - 0.04339625) < 1e-8 and abs(b[1].muaf - 0.042195) < 1e-8
+    assert abs(b[1].mutr - 1 / 48.04) < 1e-8 and b[1].g == f32(0.8) and b[1].eY == f32(0.043)
+    assert abs(b[1].albedof - 48 / 48.00339625) < 1e-7
+    assert abs(b[2].mua - 0.09698125) < 1e-7 and abs(b[2].muaf - 0.090975) < 1e-7
+    assert abs(b[2].mutr - 1 / 48.08) < 1e-8
+    # specular start weight for n 1 -> 1.33
+    r = ((1.0 - float(f32(1.33))) / (1.0 + float(f32(1.33)))) ** 2
+    assert s.start_weight == int(4294967295.0 * (1 - r))
+    # labels: the digits of the bulk file, in file order
+    digits = np.array([int(c) for c in open("3dbulk-sample").read().split()], np.uint8)
+    assert np.array_equal(sim.labels, digits)
+    # the _param.dat echo range starts at the output-name line
+    raw = open("sample2.mci", "rb").read()
+    assert raw[s.begin:].lstrip().startswith(b"sample2-out") and s.end > s.begin
+
+
+def test_parse_sample_method1(in_ref_inputs):
+    sim = M.Simulation.from_mci("sample.mci")
+    s = sim.s
+    assert s.bulk_method == 1 and s.n_layers == 1
+    assert s.number_of_photons_per_voxel == 80000
+    L = s.layers[1]
+    assert L.z_min == 0.0 and L.z_max == 5.0 and s.esp == 5.0
+    assert abs(L.mua - 0.04339625) < 1e-8
+    inc = s.inclusion
+    assert (inc.x, inc.y, inc.z) == (f32(-1.9), 0.0, f32(1.55)) and inc.r == f32(0.595)
+    assert abs(inc.mua - 0.09698125) < 1e-7 and abs(inc.mutr - 1 / 48.08) < 1e-8
+
+
+def write_mci_variant(tmp_path, **edits):
+    src = open(os.path.join(REF_INPUTS, "sample2.mci")).read()
+    for old, new in edits.items():
+        src = src.replace(old, new, 1)
+    p = tmp_path / "v.mci"
+    p.write_text(src)
+    return str(p)
+
+
+def test_parser_clamps_photon_counts(tmp_path, monkeypatch):
+    monkeypatch.chdir(REF_INPUTS)
+    p = write_mci_variant(tmp_path, **{"500000000 ": "1000 ", "40000 ": "32768 "})
+    s = M.Simulation.from_mci(p).s
+    assert s.number_of_photons == 32767 and s.number_of_photons_per_voxel == 32767
+
+
+@pytest.mark.parametrize("edit", [
+    {"1                                                                             # Do fluo": "3 # Do fluo"},
+    {"1                                                                             # Do temp": "7 # Do temp"},
+    {"0.04\t0.04": "0.04 0.04 0.04"},
+])
+def test_parser_rejects_malformed(tmp_path, monkeypatch, edit):
+    monkeypatch.chdir(REF_INPUTS)
+    p = write_mci_variant(tmp_path, **edit)
+    with pytest.raises(M.McflError):
+        M.Simulation.from_mci(p)
+
+
+def test_parser_missing_file():
+    with pytest.raises(M.McflError):
+        M.Simulation.from_mci("/nonexistent/file.mci")
+
+
+def test_dims_extension_roundtrip(tmp_path, monkeypatch):
+    from cudamcfl_amd.media import write_bulk, write_mci, cfg2_medium
+    monkeypatch.chdir(tmp_path)
+    lab, mats = cfg2_medium(seed=5, shape=(8, 12, 10))
+    write_bulk("b.txt", lab)
+    write_mci("m.mci", "out", "b.txt", mats, esp=0.4, grid_size=20, photons=10 ** 6, dims=(10, 12, 8))
+    s = M.Simulation.from_mci("m.mci")
+    assert (s.s.nx, s.s.ny, s.s.nz) == (10, 12, 8)
+    assert s.s.halfx == f32(10) / f32(40) and s.s.halfy == f32(12) / f32(40)
+    assert np.array_equal(s.labels, lab.reshape(-1))
+    b1 = s.s.bulks[1]
+    assert b1.n == f32(mats[0][0]) and b1.g == f32(mats[0][7])
+
+
+def test_interpret_arg():
+    lib = M.lib()
+    argv = (C.c_char_p * 4)(b"mcfl", b"x.mci", b"-S12345", b"-A")
+    seed, ign = C.c_uint64(0), C.c_int(0)
+    assert lib.mcfl_interpret_arg(4, argv, C.byref(seed), C.byref(ign)) == 0
+    assert seed.value == 12345 and ign.value == 1
+    argv = (C.c_char_p * 3)(b"mcfl", b"x.mci", b"-Q")
+    assert lib.mcfl_interpret_arg(3, argv, C.byref(seed), C.byref(ign)) == 1
+
+
+def test_writers_byte_layout(tmp_path, monkeypatch):
+    monkeypatch.chdir(REF_INPUTS)
+    sim = M.Simulation.from_mci("sample2.mci")
+    s = sim.s
+    s.outp_filename = str(tmp_path / "w").encode()
+    s.det.nx, s.det.ny = 3, 2
+    s.do_temp_sim = 1
+    s.det.temp_bins = 2
+    npix = 3 * 3
+    r = M.ExcitationResult(sim, fhd=False)
+    r.Rd[:] = np.arange(npix, dtype=np.uint64) * 1000003
+    r.Tt[:] = np.arange(npix, dtype=np.uint64)[::-1] * 77
+    r.time[:4] = [5, 6, 7, 8]
+    assert M.lib().mcfl_write_simulation_results(C.byref(s), C.byref(r.t), 1.25) == 0
+    scale = float(0xFFFFFFFF) * 500000000.0
+    scale2 = scale * float(f32(0.04)) * float(f32(0.04))
+
+    def img(a):
+        return "".join("".join(" %E " % (float(a[y * 3 + x]) / scale2) for x in range(3)) + " \n " for y in range(2))
+
+    assert open(tmp_path / "w_Refl.dat").read() == img(r.Rd)
+    assert open(tmp_path / "w_Trans.dat").read() == img(r.Tt)
+    t = open(tmp_path / "w_time.dat").read()
+    assert t.startswith("#x y timebins[numbins]\n")
+    assert "%f %f " % (-0.5, 0.0) + " %E " % (5 / scale) + " %E " % (7 / scale) + " \n " in t
+    p = open(tmp_path / "w_param.dat", "rb").read()
+    assert b"# User time: 1.25 sec" in p
+    raw = open("sample2.mci", "rb").read()
+    assert p.split(b"InParam\t\t# Input parameters:\n", 1)[1] == raw[s.begin:s.end]
+
+
+def test_write_phd_and_fl(tmp_path):
+    lab = np.ones((2, 2, 3), np.uint8)
+    sim = M.Simulation.voxel(lab, [[1.4, 0.1, 10, 0, 0, 0, 0, 0.9]], esp=0.2, grid_size=10,
+                             det=(0.1, 0.1, 2, 2, 0, 0), outp=str(tmp_path / "p"))
+    Fx = np.arange(12, dtype=np.float64) * 1.5e-3
+    assert M.lib().mcfl_write_phd(C.byref(sim.s), Fx.ctypes.data_as(M.F64P)) == 0
+    txt = open(tmp_path / "p_PHD-Ascii.dat").read()
+    assert txt == "3\t2\t2\n" + "".join("%.10E\n" % v for v in Fx)
+    R = np.array([1.0, 2.5e-7, 3.0, 0.0])
+    T = np.array([0.5, 0.25, 0.125, 1e-12])
+    assert M.lib().mcfl_write_fl(C.byref(sim.s), R.ctypes.data_as(M.F64P), T.ctypes.data_as(M.F64P)) == 0
+    assert open(tmp_path / "p_FlR.dat").read() == " %E  %E  \n  %E  %E  \n " % tuple(R)
+    assert open(tmp_path / "p_FlT.dat").read() == " %E  %E  \n  %E  %E  \n " % tuple(T)
+
+
+def test_fl_plan_matches_reference_weights():
+    import media
+    sim = media.small_sim(photons=1000)
+    rng = np.random.default_rng(0)
+    Fx = rng.uniform(0, 1e-3, sim.nvox)
+    n = 50000
+    vox, scale, unit = M.fl_plan(sim, Fx, n)
+    lab = sim.labels
+    bul = [sim.s.bulks[i] for i in range(sim.s.n_bulks + 2)]
+    exp_vox = [v for v in range(sim.nvox) if bul[lab[v]].albedof < 1]
+    assert np.array_equal(vox, exp_vox)
+    voxelw = np.array([bul[lab[v]].eY * float(f32(1) - f32(bul[lab[v]].albedof)) * Fx[v] / (n * 4294967295.0)
+                       for v in vox])
+    np.testing.assert_allclose(scale * unit, voxelw, rtol=1e-14)
+    assert unit > 0 and (voxelw * n * 2.0 ** 32).sum() / unit <= 2.0 ** 62 * (1 + 1e-12)
+
+
+def test_cli_without_arguments():
+    exe = os.path.join(M.BUILD_DIR, "mcfl")
+    p = subprocess.run([exe], capture_output=True, text=True)
+    assert p.returncode == 1 and "Not enough input arguments!" in p.stdout
+
+
+def test_cli_rejects_unknown_argument(monkeypatch):
+    monkeypatch.chdir(REF_INPUTS)
+    exe = os.path.join(M.BUILD_DIR, "mcfl")
+    p = subprocess.run([exe, "sample2.mci", "-Z"], capture_output=True, text=True)
+    assert p.returncode == 1 and "Unknown argument -Z!" in p.stdout

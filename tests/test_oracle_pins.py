@@ -1,0 +1,352 @@
+"""Pins of the CPU oracle and of the shared FP32 math / MWC headers.
+
+Each check comes from outside the code it tests: libm in double precision,
+the MWC closed form (n-th state = a^n x0 mod a*2^32-1, Python big integers),
+textbook Fresnel, Henyey-Greenstein moments, the reference notebook's
+back-step known answers (FoundBoundaryPython.ipynb), van de Hulst's published
+slab values, and Chandrasekhar's H-function for a semi-infinite isotropic
+medium.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import cudamcfl_amd as M
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+@pytest.fixture(scope="module")
+def a_tab():
+    return M.multipliers(32768)
+
+
+# ------------------------------------------------------------ math header
+
+def test_log_error_bound():
+    rng = np.random.default_rng(0)
+    u = np.concatenate([rng.uniform(0, 1, 200000), np.geomspace(2.0 ** -24, 1.0, 50000),
+                        1.0 - np.arange(1, 20000) * 2.0 ** -24]).astype(np.float32)
+    u = u[u > 0]
+    got = O.math_eval(0, u).astype(np.float64)
+    ref = np.log(u.astype(np.float64))
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    err = np.abs(got - ref)
+    nz = ref != 0
+    assert np.all(err[nz] <= 2.0 * ulp[nz]), f"max ulp error {np.max(err[nz] / ulp[nz])}"
+    assert O.math_eval(0, np.array([1.0], np.float32))[0] == 0.0
+
+
+def test_sincos_turn_error_bound():
+    rng = np.random.default_rng(1)
+    t = np.concatenate([rng.uniform(0, 1, 200000), np.linspace(0, 1, 4097)]).astype(np.float32)
+    s = O.math_eval(1, t).astype(np.float64)
+    c = O.math_eval(2, t).astype(np.float64)
+    ang = 2 * np.pi * t.astype(np.float64)
+    assert np.max(np.abs(s - np.sin(ang))) < 1.5e-7
+    assert np.max(np.abs(c - np.cos(ang))) < 1.5e-7
+    # quadrant points are exact
+    q = np.array([0, 0.25, 0.5, 0.75], np.float32)
+    assert np.array_equal(np.abs(O.math_eval(1, q)), [0, 1, 0, 1])
+    assert np.array_equal(np.abs(O.math_eval(2, q)), [1, 0, 1, 0])
+
+
+def test_u32_to_float_toward_zero():
+    rng = np.random.default_rng(2)
+    u = np.concatenate([rng.integers(0, 2 ** 32, 100000, dtype=np.uint64),
+                        [0, 1, 2 ** 24 - 1, 2 ** 24, 2 ** 24 + 1, 2 ** 32 - 1, 2 ** 31 + 255]]).astype(np.uint32)
+    got = O.math_eval(3, u.view(np.float32))
+    f = u.astype(np.float32)  # nearest
+    over = f.astype(np.float64) > u.astype(np.float64)
+    f[over] = np.nextafter(f[over], np.float32(0))
+    assert np.array_equal(got, f)
+
+
+# ------------------------------------------------------------- generator
+
+def test_mwc_closed_form():
+    a = 4294966893
+    p = a * 2 ** 32 - 1
+    x0 = (123456 << 32) | 987654321
+    states, co = O.mwc_draws(x0, a, 5000)
+    for n in (1, 2, 3, 17, 1000, 4999):
+        assert int(states[n - 1]) == pow(a, n, p) * x0 % p
+    lo = (states & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    f = lo.astype(np.float32)
+    over = f.astype(np.float64) > lo.astype(np.float64)
+    f[over] = np.nextafter(f[over], np.float32(0))
+    assert np.array_equal(co, f * np.float32(2.0 ** -32))
+    assert co.min() >= 0 and co.max() < 1
+
+
+def test_mwc_long_jump():
+    a = 4294967118
+    p = a * 2 ** 32 - 1
+    x0 = 77 << 32 | 5
+    states, _ = O.mwc_draws(x0, a, 1_000_000)
+    assert int(states[-1]) == pow(a, 1_000_000, p) * x0 % p
+
+
+def test_primes_table_rule():
+    path = M.PRIMES_PATH
+    col = np.loadtxt(path, dtype=np.uint64, usecols=0)
+    assert len(col) == 32769
+    assert np.all(np.diff(col.astype(np.int64)) < 0), "descending order"
+
+    def is_prime(n):
+        if n < 2:
+            return False
+        for sp in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+            if n % sp == 0:
+                return n == sp
+        d, s = n - 1, 0
+        while d % 2 == 0:
+            d //= 2
+            s += 1
+        for b in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+            x = pow(b, d, n)
+            if x in (1, n - 1):
+                continue
+            for _ in range(s - 1):
+                x = x * x % n
+                if x == n - 1:
+                    break
+            else:
+                return False
+        return True
+
+    idx = list(range(0, 20)) + list(range(len(col) - 20, len(col))) + list(range(100, len(col), 997))
+    for i in idx:
+        a = int(col[i])
+        assert is_prime(a * 2 ** 32 - 1) and is_prime(a * 2 ** 31 - 1), a
+    # no qualifying multiplier was skipped between the first two lines
+    a0, a1 = int(col[0]), int(col[1])
+    for a in range(a1 + 1, a0):
+        assert not (is_prime(a * 2 ** 31 - 1) and is_prime(a * 2 ** 32 - 1))
+    for a in range(a0 + 1, 2 ** 32):
+        assert not (is_prime(a * 2 ** 31 - 1) and is_prime(a * 2 ** 32 - 1))
+
+
+def test_init_rng_matches_seeding_closed_form():
+    n = 300
+    seed = (4567 << 32) | 891011
+    rc, x, a = O.init_rng(n, M.PRIMES_PATH, seed)
+    assert rc == 0
+    col = np.loadtxt(M.PRIMES_PATH, dtype=np.uint64, usecols=0, max_rows=n + 1)
+    begin = int(col[0])
+    pb = begin * 2 ** 32 - 1
+    assert np.array_equal(a, col[1:].astype(np.uint32))
+    k = 0
+    for i in range(n):
+        # two seeding draws per lane unless a state was rejected (then more)
+        while True:
+            u1 = (pow(begin, k + 1, pb) * seed % pb) & 0xFFFFFFFF
+            u2 = (pow(begin, k + 2, pb) * seed % pb) & 0xFFFFFFFF
+            k += 2
+            c = math.floor((float(u1) / 4294967296.0) * int(a[i]))
+            xi = (c << 32) + u2
+            if not (xi == 0 or c >= int(a[i]) - 1 or u2 >= 0xFFFFFFFF):
+                break
+        assert int(x[i]) == xi, i
+
+
+def test_init_rng_product_equals_oracle():
+    for seed in (1, 2 ** 40 + 12345, 0xFFFFFFF0):
+        x, a = M.init_rng(32768, seed=seed)
+        rc, xo, ao = O.init_rng(32768, M.PRIMES_PATH, seed)
+        assert rc == 0 and np.array_equal(x, xo) and np.array_equal(a, ao)
+
+
+@pytest.mark.parametrize("seed", [0, (2 ** 32 - 1), (4294967117 << 32) | 5])
+def test_init_rng_rejects_invalid_seeds(seed):
+    rc, _, _ = O.init_rng(4, M.PRIMES_PATH, seed)
+    assert rc == 1
+    with pytest.raises(M.McflError):
+        M.init_rng(4, seed=seed)
+
+
+def test_photon_states_valid_and_distinct(a_tab):
+    key = M.stream_key(42, 0)
+    seen = set()
+    for k in range(20000):
+        a = int(a_tab[k & 32767])
+        s = O.photon_state(key, k, a)
+        c, lo = s >> 32, s & 0xFFFFFFFF
+        assert c < a - 1 and lo != 0xFFFFFFFF and s != 0
+        seen.add(s)
+    assert len(seen) == 20000
+
+
+# ------------------------------------------------------ physics pieces
+
+def fresnel_textbook(n1, n2, ci):
+    ti = math.acos(ci)
+    st = n1 / n2 * math.sin(ti)
+    tt = math.asin(st)
+    a, b = ti - tt, ti + tt
+    return 0.5 * (math.sin(a) ** 2 / math.sin(b) ** 2 + math.tan(a) ** 2 / math.tan(b) ** 2)
+
+
+@pytest.mark.parametrize("n1,n2", [(1.4, 1.0), (1.0, 1.4), (1.33, 1.5), (1.5, 1.33)])
+def test_fresnel_closed_form(n1, n2):
+    lib = O.lib()
+    for deg in np.linspace(2, 88, 44):
+        ci = math.cos(math.radians(deg))
+        if n1 > n2 and math.sin(math.radians(deg)) >= n2 / n1:
+            continue
+        r = lib.orc_fresnel_test(n1, n2, ci)
+        assert abs(r - fresnel_textbook(n1, n2, ci)) < 2e-5 + 1e-4 * fresnel_textbook(n1, n2, ci), deg
+
+
+def test_reflect_branches():
+    import ctypes as C
+    lib = O.lib()
+    dz, dxs = C.c_float(), C.c_float()
+    # total internal reflection: 72.5 deg from the normal, 1.5 -> 1.0
+    assert lib.orc_reflect_test(1.5, 1.0, 0.3, 0.99, C.byref(dz), C.byref(dxs)) == 1 and abs(dz.value + 0.3) < 1e-7
+    # normal incidence: R = ((1.5-1)/(2.5))^2 = 0.04
+    assert lib.orc_reflect_test(1.5, 1.0, 1.0, 0.03, C.byref(dz), C.byref(dxs)) == 1
+    assert lib.orc_reflect_test(1.5, 1.0, 1.0, 0.05, C.byref(dz), C.byref(dxs)) == 0 and dz.value == 1.0
+    # matched index: always transmitted, direction unchanged
+    assert lib.orc_reflect_test(1.4, 1.4, 0.5, 0.0, C.byref(dz), C.byref(dxs)) == 0 and dz.value == 0.5
+    # oblique transmission: Snell's law on the tangential component
+    r = lib.orc_reflect_test(1.0, 1.5, 0.8, 0.999, C.byref(dz), C.byref(dxs))
+    assert r == 0 and abs(dxs.value - 1.0 / 1.5) < 1e-6
+    st = math.sqrt(1 - 0.8 ** 2) / 1.5
+    assert abs(dz.value - math.sqrt(1 - st * st)) < 1e-6
+
+
+@pytest.mark.parametrize("g", [0.0, 0.5, 0.9])
+def test_spin_henyey_greenstein_moments(g):
+    import ctypes as C
+    lib = O.lib()
+    n = 100000
+    x = C.c_uint64((9 << 32) | 1234567)
+    a = 4294966893
+    din = np.array([0.6, 0.0, 0.8], np.float32)
+    dout = np.zeros(3, np.float32)
+    mu = np.empty(n)
+    norms = np.empty(n)
+    for i in range(n):
+        lib.orc_spin_test(g, O.ptr(din, O.F32P), C.byref(x), a, O.ptr(dout, O.F32P))
+        mu[i] = float(np.dot(din.astype(np.float64), dout.astype(np.float64)))
+        norms[i] = float(np.linalg.norm(dout.astype(np.float64)))
+    assert np.max(np.abs(norms - 1)) < 1e-6
+    assert abs(mu.mean() - g) < 4 * mu.std() / math.sqrt(n) + 1e-4
+    m2 = (2 * g * g + 1) / 3
+    assert abs((mu ** 2).mean() - m2) < 4 * (mu ** 2).std() / math.sqrt(n) + 1e-4
+
+
+def test_backstep_known_answers():
+    import ctypes as C
+    lib = O.lib()
+    t1, t2 = C.c_float(), C.c_float()
+    f = lambda v: np.array(v, np.float32)
+    lib.orc_backstep_test(O.ptr(f([0.8, 1.2, 1]), O.F32P), O.ptr(f([0, 0, 1]), O.F32P), O.ptr(f([1, 1, 1]), O.F32P),
+                          0.25, C.byref(t1), C.byref(t2))
+    assert abs(t1.value + 0.412310562562) < 1e-6 and abs(t2.value - 0.412310562562) < 1e-6
+    r = np.float32(0.141421)
+    lib.orc_backstep_test(O.ptr(f([1.489575, -0.333571, 1.599638]), O.F32P),
+                          O.ptr(f([0.422606, -0.768113, -0.481048]), O.F32P),
+                          O.ptr(f([1.4, -0.4, 1.6]), O.F32P), float(r * r), C.byref(t1), C.byref(t2))
+    assert abs(t1.value + 0.10092855989933014) < 2e-6 and abs(t2.value - 0.07493677735328674) < 2e-6
+
+
+# -------------------------------------------------- whole-transport pins
+
+def homogeneous(mua, mus, g, d, gs, n_in=1.0, n_out=1.0):
+    nz = int(round(d * gs))
+    nx = int(4 * np.float32(d) * gs)
+    lab = np.ones((nz, nx, nx), np.uint8)
+    f32 = np.float32
+    b = [[FLT_MAX, 0, 1, n_out, 0, 0, 0, 0],
+         [f32(1) / f32(mua + mus), mua, g, n_in, 0, mua, 0, 1],
+         [FLT_MAX, 0, 1, n_out, 0, 0, 0, 0]]
+    return O.Medium(lab, b, d, gs)
+
+
+def test_van_de_hulst_slab(a_tab):
+    """mua=10, mus=90 /cm, g=0.75, n=1, d=0.02 cm: Rd = 0.09739, Tt = 0.66096
+    (van de Hulst; MCML's validation table)."""
+    m = homogeneous(10, 90, 0.75, 0.02, 50)
+    N = 300000
+    r = O.run_photons(m.sim(src=(1, 0, 0, 0, 0)), a_tab, 2024, 0, N)
+    W = N * 0xFFFFFFFF
+    t = r.tot
+    assert abs(t["R"] / W - 0.09739) < 0.0025
+    assert abs(t["T"] / W - 0.66096) < 0.003
+
+
+def chandrasekhar_rd(albedo, mu0=1.0, n=64):
+    x, w = np.polynomial.legendre.leggauss(n)
+    mu = 0.5 * (x + 1)
+    w = 0.5 * w
+    H = np.ones(n)
+    for _ in range(500):
+        Hn = 1.0 / (math.sqrt(1 - albedo) + 0.5 * albedo * (w * mu * H / (mu[:, None] + mu)).sum(axis=1))
+        if np.max(np.abs(Hn - H)) < 1e-14:
+            break
+        H = Hn
+    H0 = 1.0 / (math.sqrt(1 - albedo) + 0.5 * albedo * np.sum(w * mu * H / (mu0 + mu)))
+    return 1 - H0 * math.sqrt(1 - albedo)
+
+
+def test_semi_infinite_isotropic_chandrasekhar(a_tab):
+    """Isotropic scattering, albedo 0.9, matched index, normal incidence:
+    R = 1 - H(1) sqrt(1 - albedo)."""
+    m = homogeneous(10, 90, 0.0, 1.0, 5)
+    N = 200000
+    r = O.run_photons(m.sim(src=(1, 0, 0, 0, 0)), a_tab, 77, 0, N)
+    t = r.tot
+    R = t["R"] / (N * 0xFFFFFFFF)
+    assert t["T"] == 0
+    assert abs(R - chandrasekhar_rd(0.9)) < 0.006, (R, chandrasekhar_rd(0.9))
+
+
+def test_energy_balance_and_counts(a_tab):
+    import media
+    sim = media.small_sim(photons=3000)
+    o = O.FromProduct(sim, axyz=True)
+    r = O.run_photons(o.o, a_tab, 5, 0, 3000)
+    t = r.tot
+    assert t["photons"] == 3000 and t["aborted"] == 0
+    assert t["launched_w"] == 3000 * sim.s.start_weight
+    assert t["launched_w"] + t["roul_gain"] - t["roul_loss"] - t["zero_kill"] == t["R"] + t["T"] + t["A"]
+    # absorption laterally outside the grid (medium 1, infinite) has no voxel
+    assert 0.5 * t["A"] < int(r.axyz.sum()) <= t["A"]
+    assert int(r.Rd.sum()) <= t["R"] and int(r.Tt.sum()) <= t["T"]
+
+
+def test_thread_count_does_not_change_tallies(a_tab):
+    import media
+    sim = media.small_sim(photons=2000)
+    o = O.FromProduct(sim)
+    r1 = O.run_photons(o.o, a_tab, 9, 0, 2000, nthreads=1)
+    r8 = O.run_photons(o.o, a_tab, 9, 0, 2000, nthreads=8)
+    assert np.array_equal(r1.Rd, r8.Rd) and np.array_equal(r1.fhd, r8.fhd) and r1.tot == r8.tot
+
+
+def test_lane_and_photon_streams_agree_statistically(a_tab):
+    """The reference's per-lane streams and this build's per-photon streams
+    sample the same transport: totals agree within Monte Carlo noise."""
+    import media
+    N = 40000
+    sim = media.small_sim(photons=N, temporal=False)
+    o = O.FromProduct(sim)
+    rp = O.run_photons(o.o, a_tab, 31, 0, N)
+    x, a = M.init_rng(1024, seed=99)
+    rl = O.run_lanes(o.o, x, a, N)
+    for k in ("R", "T", "A"):
+        p, l = rp.tot[k] / rp.tot["launched_w"], rl.tot[k] / rl.tot["launched_w"]
+        assert abs(p - l) < 5 * math.sqrt(max(p, 1e-3) / N), (k, p, l)
+
+
+def test_lanes_require_enough_photons(a_tab):
+    import media
+    sim = media.small_sim(photons=100)
+    o = O.FromProduct(sim)
+    x, a = M.init_rng(512, seed=3)
+    with pytest.raises(ValueError):
+        O.run_lanes(o.o, x, a, 100)
