@@ -78,6 +78,8 @@ def main():
         r = ctx.run_excitation(sim, seed=k + 1)
         walls.append(time.perf_counter() - t0)
         kms.append(r.kernel_ms)
+        print(f"step {k}: kernel {r.kernel_ms:.1f} ms, {N / r.kernel_ms * 1e3:.3e} photons/s", file=sys.stderr,
+              flush=True)
         steps += r.totals["steps"]
         t = r.totals
         assert t["photons"] == N

@@ -356,3 +356,183 @@ def fl_plan(sim, Fx, n_per_voxel):
                              C.byref(cap), C.byref(unit)), "fl_plan")
     n = cap.value
     return vox[:n].copy(), scale[:n].copy(), unit.value
+
+
+# ------------------------------------------------------------ batch / serving
+
+SOURCE_EMISSION = 3
+TALLY_FHD, TALLY_SQ, TALLY_TEMP, TALLY_AXYZ = 1, 2, 4, 8
+MAX_BATCH_JOBS = 64
+
+
+class Job(C.Structure):
+    _fields_ = [("medium", C.c_int32), ("src_kind", C.c_int32), ("xi", C.c_float), ("yi", C.c_float),
+                ("zi", C.c_float), ("src_r", C.c_float), ("photons", C.c_uint64), ("seed", C.c_uint64),
+                ("tallies", C.c_uint32), ("Fx", F64P),
+                ("Rd_xy", U64P), ("Tt_xy", U64P), ("fhd", U64P), ("A_xyz", U64P), ("Rd2_xy", U64P),
+                ("Tt2_xy", U64P), ("time_xyt", U64P), ("totals", C.c_uint64 * NTOT), ("fl_unit", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("medium", C.c_int32), ("src_kind", C.c_int32), ("xi", C.c_float), ("yi", C.c_float),
+                ("zi", C.c_float), ("src_r", C.c_float), ("photons", C.c_uint64), ("seed", C.c_uint64),
+                ("fluorescence", C.c_int32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("Rd_xy", U64P), ("Tt_xy", U64P), ("FlR_raw", U64P), ("FlT_raw", U64P), ("fl_unit", C.c_double),
+                ("totals", C.c_uint64 * NTOT), ("fl_totals", C.c_uint64 * NTOT), ("epochs", C.c_uint64)]
+
+
+def _bind_engine(L):
+    if getattr(L, "_engine", False):
+        return
+    SP = C.POINTER(C.POINTER(Sim))
+    L.mcfl_batch_create.argtypes = [C.c_void_p, SP, C.c_int, C.POINTER(Job), C.c_int, C.POINTER(C.c_void_p)]
+    L.mcfl_batch_run.argtypes = [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_double)]
+    L.mcfl_batch_destroy.argtypes = [C.c_void_p]
+    L.mcfl_batch_destroy.restype = None
+    L.mcfl_run_batch.argtypes = [C.c_void_p, SP, C.c_int, C.POINTER(Job), C.c_int, C.POINTER(C.c_double)]
+    L.mcfl_server_create.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]
+    L.mcfl_server_add_medium.argtypes = [C.c_void_p, C.POINTER(Sim), C.POINTER(C.c_int32)]
+    L.mcfl_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]
+    L.mcfl_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double)]
+    L.mcfl_server_collect.argtypes = [C.c_void_p, C.c_int64, C.POINTER(RequestResult)]
+    L.mcfl_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
+    L.mcfl_server_destroy.argtypes = [C.c_void_p]
+    L.mcfl_server_destroy.restype = None
+    L._engine = True
+
+
+EXPORTS += ["mcfl_batch_create", "mcfl_batch_run", "mcfl_batch_destroy", "mcfl_run_batch", "mcfl_server_create",
+            "mcfl_server_add_medium", "mcfl_server_submit", "mcfl_server_step", "mcfl_server_collect",
+            "mcfl_server_stats", "mcfl_server_destroy"]
+
+
+class BatchJob:
+    """One simulation of a batch: source, budget, seed, tallies; outputs as numpy arrays."""
+
+    def __init__(self, medium, photons, seed, source=(0.0, 0.0, 0.0), src_kind=SOURCE_PENCIL, src_r=0.0,
+                 tallies=0, Fx=None):
+        self.medium, self.photons, self.seed = medium, int(photons), int(seed)
+        self.source, self.src_kind, self.src_r, self.tallies = source, src_kind, src_r, tallies
+        self.Fx = None if Fx is None else np.ascontiguousarray(Fx, np.float64)
+
+    def make(self, sims):
+        s = sims[self.medium]
+        j = Job()
+        j.medium, j.src_kind = self.medium, self.src_kind
+        j.xi, j.yi, j.zi = self.source
+        j.src_r, j.photons, j.seed, j.tallies = self.src_r, self.photons, self.seed, self.tallies
+        j.Fx = _p(self.Fx, F64P)
+        self.Rd = np.zeros(s.npix, np.uint64)
+        self.Tt = np.zeros(s.npix, np.uint64)
+        self.fhd = np.zeros(s.nvox, np.uint64) if self.tallies & TALLY_FHD else None
+        self.A_xyz = np.zeros(s.nvox, np.uint64) if self.tallies & TALLY_AXYZ else None
+        self.Rd2 = np.zeros(s.npix, np.uint64) if self.tallies & TALLY_SQ else None
+        self.Tt2 = np.zeros(s.npix, np.uint64) if self.tallies & TALLY_SQ else None
+        nt = s.s.det.x_temp_numdets * s.s.det.y_temp_numdets * s.s.det.temp_bins
+        self.time = np.zeros(max(nt, 1), np.uint64) if self.tallies & TALLY_TEMP else None
+        j.Rd_xy, j.Tt_xy, j.fhd, j.A_xyz = _p(self.Rd, U64P), _p(self.Tt, U64P), _p(self.fhd, U64P), _p(self.A_xyz, U64P)
+        j.Rd2_xy, j.Tt2_xy, j.time_xyt = _p(self.Rd2, U64P), _p(self.Tt2, U64P), _p(self.time, U64P)
+        return j
+
+
+class Batch:
+    """mcfl_batch: media + jobs captured once in a HIP graph, replayed by run()."""
+
+    def __init__(self, ctx, sims, jobs):
+        L = lib()
+        _bind_engine(L)
+        self.sims, self.jobs = list(sims), list(jobs)
+        self._sarr = (C.POINTER(Sim) * len(self.sims))(*[C.pointer(s.s) for s in self.sims])
+        self._jarr = (Job * len(self.jobs))(*[j.make(self.sims) for j in self.jobs])
+        self.h = C.c_void_p()
+        check(L.mcfl_batch_create(ctx.h, self._sarr, len(self.sims), self._jarr, len(self.jobs), C.byref(self.h)),
+              "mcfl_batch_create")
+
+    def run(self, seeds=None):
+        if seeds is not None:
+            for j, s in zip(self._jarr, seeds):
+                j.seed = int(s)
+        ms = C.c_double()
+        check(lib().mcfl_batch_run(self.h, self._jarr, C.byref(ms)), "mcfl_batch_run")
+        for j, jj in zip(self.jobs, self._jarr):
+            j.totals = {n: int(jj.totals[i]) for i, n in enumerate(TOT_NAMES)}
+            j.fl_unit = jj.fl_unit
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().mcfl_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """mcfl_server: continuous batching of forward-simulation requests."""
+
+    def __init__(self, ctx, mem_budget=8 << 30, epoch_photons=2_000_000):
+        L = lib()
+        _bind_engine(L)
+        self.h = C.c_void_p()
+        check(L.mcfl_server_create(ctx.h, int(mem_budget), int(epoch_photons), C.byref(self.h)), "mcfl_server_create")
+        self.media = []
+
+    def add_medium(self, sim):
+        mid = C.c_int32()
+        check(lib().mcfl_server_add_medium(self.h, C.byref(sim.s), C.byref(mid)), "add_medium")
+        self.media.append(sim)
+        return mid.value
+
+    def submit(self, medium, photons, seed, source=(0.0, 0.0, 0.0), src_kind=SOURCE_PENCIL, src_r=0.0,
+               fluorescence=False):
+        r = Request()
+        r.medium, r.src_kind = medium, src_kind
+        r.xi, r.yi, r.zi = source
+        r.src_r, r.photons, r.seed, r.fluorescence = src_r, int(photons), int(seed), int(bool(fluorescence))
+        rid = C.c_int64()
+        check(lib().mcfl_server_submit(self.h, C.byref(r), C.byref(rid)), "submit")
+        return rid.value
+
+    def step(self, max_done=256):
+        ids = (C.c_int64 * max_done)()
+        nd, ms = C.c_int32(), C.c_double()
+        check(lib().mcfl_server_step(self.h, ids, max_done, C.byref(nd), C.byref(ms)), "step")
+        return [ids[i] for i in range(min(nd.value, max_done))], ms.value
+
+    def collect(self, rid, medium):
+        s = self.media[medium]
+        out = {"Rd": np.zeros(s.npix, np.uint64), "Tt": np.zeros(s.npix, np.uint64),
+               "FlR": np.zeros(s.npix, np.uint64), "FlT": np.zeros(s.npix, np.uint64)}
+        r = RequestResult()
+        r.Rd_xy, r.Tt_xy = _p(out["Rd"], U64P), _p(out["Tt"], U64P)
+        r.FlR_raw, r.FlT_raw = _p(out["FlR"], U64P), _p(out["FlT"], U64P)
+        check(lib().mcfl_server_collect(self.h, rid, C.byref(r)), "collect")
+        out["fl_unit"] = r.fl_unit
+        out["totals"] = {n: int(r.totals[i]) for i, n in enumerate(TOT_NAMES)}
+        out["fl_totals"] = {n: int(r.fl_totals[i]) for i, n in enumerate(TOT_NAMES)}
+        out["epochs"] = int(r.epochs)
+        return out
+
+    def stats(self):
+        q, a, b = C.c_int32(), C.c_int32(), C.c_uint64()
+        check(lib().mcfl_server_stats(self.h, C.byref(q), C.byref(a), C.byref(b)), "stats")
+        return {"queued": q.value, "active": a.value, "bytes": b.value}
+
+    def close(self):
+        if self.h:
+            lib().mcfl_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -14,13 +14,13 @@
 
 #include <stdint.h>
 
-#define MCFL_MAX_MATERIALS 128   /* labels are 8-bit; the reference caps bulks at 100 (CUDAMCFL.h:24) */
+#define MCFL_MAX_MATERIALS 320   /* all material tables of one launch (LDS); one medium holds at most 100 (CUDAMCFL.h:24) */
 #define MCFL_MAX_JOBS 64         /* explicit jobs per launch (per-job totals live in LDS) */
 #define MCFL_NTOT 16             /* totals per job, see enum mcfl_total in mcfl.h */
 #define MCFL_MAX_STEPS_PER_PHOTON (1u << 28)
 
 enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2 };
-enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2 };
+enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2, MCFL_SRC_EMISSION = 3 };
 enum {
   MCFL_DT_LAUNCHED_W = 0,
   MCFL_DT_R,
@@ -72,6 +72,7 @@ struct JobDev {
   uint64_t begin;  /* first flattened photon index */
   uint64_t count;  /* photons */
   uint64_t key;    /* photon-stream key */
+  uint64_t k0;     /* photon number of the job's first index (epoch slices) */
   int32_t src_kind;
   float xi, yi, zi, src_r;
   uint32_t start_weight;
@@ -87,6 +88,12 @@ struct JobDev {
   unsigned long long* Tt2;
   unsigned long long* tot; /* MCFL_NTOT counters */
   DetDev det;
+  MediumDev med; /* used when the launch mixes media */
+  /* MCFL_SRC_EMISSION: Walker alias table over emitting voxels */
+  const uint32_t* al_prob; /* acceptance threshold, 32-bit fixed point */
+  const uint32_t* al_alias;
+  const uint32_t* al_vox;
+  uint32_t al_n;
 };
 
 struct LaunchArgs {

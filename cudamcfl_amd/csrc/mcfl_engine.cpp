@@ -1,0 +1,658 @@
+// mcfl_engine.cpp -- batched launches and the continuous-batching server
+// (include/mcfl.h, mcfl_batch_* and mcfl_server_*).
+//
+// No counterpart in the reference: its multi-source scans are one process per
+// .mci file (ScriptGenerator-CeCAR-Translation.ipynb:48-55) and its emission
+// stage is a host loop of one full simulation per voxel (CUDAMCFLmain.cu:
+// 471-594).  Here every simulation of a batch or of a serving epoch is a job
+// of a single persistent launch (mcfl_transport.hip, MCFL_WORK_JOBS).
+//
+// Photon k of a job draws from the stream keyed by (seed, phase) -- phase 0
+// excitation, phase 1 emission -- whatever launch, epoch or lane runs it, so a
+// job's tallies equal those of the same job run alone.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "mcfl_internal.h"
+#include "mcfl_mwc.h"
+
+using namespace mcfl_detail;
+
+namespace {
+
+// A medium resident in device memory.
+struct Medium {
+  mcfl_sim sim{};                 // scalar fields (pointers below)
+  std::vector<mcfl_bulk> bulks;
+  std::vector<uint8_t> labels;
+  std::vector<MatDev> mat_ex, mat_em;
+  DevBuf d_labels, d_tpos;
+  size_t nvox = 0, npix = 0;
+  ~Medium() {
+    release(d_labels);
+    release(d_tpos);
+  }
+};
+
+mcfl_status medium_make(const mcfl_sim* s, Medium* m) {
+  MCFL_STCHK(validate(s));
+  m->sim = *s;
+  m->nvox = nvox_of(s);
+  m->npix = npix_of(s);
+  m->bulks.assign(s->bulks, s->bulks + s->n_bulks + 2);
+  m->labels.assign(s->bulk_info, s->bulk_info + m->nvox);
+  m->sim.bulks = m->bulks.data();
+  m->sim.bulk_info = m->labels.data();
+  m->sim.layers = nullptr;
+  build_mats(&m->sim, false, m->mat_ex);
+  build_mats(&m->sim, true, m->mat_em);
+  MCFL_STCHK(ensure(m->d_labels, m->nvox));
+  MCFL_HIPCHK(hipMemcpy(m->d_labels.p, m->labels.data(), m->nvox, hipMemcpyHostToDevice));
+  if (s->do_temp_sim == 1) {
+    std::vector<float> pos;
+    tdet_positions(s, pos);
+    MCFL_STCHK(ensure(m->d_tpos, pos.size() * sizeof(float)));
+    MCFL_HIPCHK(hipMemcpy(m->d_tpos.p, pos.data(), pos.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return MCFL_OK;
+}
+
+// Walker/Vose alias table; column i accepts itself when a uniform 32-bit
+// draw u < prob[i], else takes alias[i].
+void build_alias(const std::vector<double>& w, std::vector<uint32_t>& prob, std::vector<uint32_t>& alias) {
+  const size_t n = w.size();
+  prob.assign(n, 0xFFFFFFFFu);
+  alias.resize(n);
+  double sum = 0.0;
+  for (double v : w) sum += v;
+  std::vector<double> p(n);
+  std::vector<uint32_t> small, large;
+  for (size_t i = 0; i < n; ++i) {
+    p[i] = w[i] * (double)n / sum;
+    alias[i] = (uint32_t)i;
+    (p[i] < 1.0 ? small : large).push_back((uint32_t)i);
+  }
+  while (!small.empty() && !large.empty()) {
+    uint32_t s = small.back(), l = large.back();
+    small.pop_back();
+    alias[s] = l;
+    prob[s] = (uint32_t)std::min(4294967295.0, std::floor(p[s] * 4294967296.0));
+    p[l] = (p[l] + p[s]) - 1.0;
+    if (p[l] < 1.0) {
+      large.pop_back();
+      small.push_back(l);
+    }
+  }
+  // leftovers are full columns (rounding residue)
+}
+
+// Device tallies of one job (or one phase of a served request).
+struct Tallies {
+  DevBuf Rd, Tt, fhd, axyz, Rd2, Tt2, time, tot, al_prob, al_alias, al_vox;
+  uint32_t flags = 0;
+  uint32_t al_n = 0;
+  double fl_unit = 0.0;
+  size_t bytes = 0;
+  ~Tallies() {
+    for (DevBuf* b : {&Rd, &Tt, &fhd, &axyz, &Rd2, &Tt2, &time, &tot, &al_prob, &al_alias, &al_vox}) release(*b);
+  }
+};
+
+mcfl_status alloc_tallies(const Medium& m, uint32_t tallies, bool emission, Tallies* t) {
+  const size_t npix = m.npix * 8, nvox = m.nvox * 8;
+  MCFL_STCHK(ensure(t->Rd, npix));
+  MCFL_STCHK(ensure(t->Tt, npix));
+  MCFL_STCHK(ensure(t->tot, MCFL_NTOT * 8));
+  t->bytes = 2 * npix + MCFL_NTOT * 8;
+  t->flags = 0;
+  if ((tallies & MCFL_TALLY_FHD) && !emission) {
+    MCFL_STCHK(ensure(t->fhd, nvox));
+    t->flags |= MCFL_F_FHD;
+    t->bytes += nvox;
+  }
+  if (tallies & MCFL_TALLY_AXYZ) {
+    MCFL_STCHK(ensure(t->axyz, nvox));
+    t->flags |= MCFL_F_AXYZ;
+    t->bytes += nvox;
+  }
+  if (tallies & MCFL_TALLY_SQ) {
+    MCFL_STCHK(ensure(t->Rd2, npix));
+    MCFL_STCHK(ensure(t->Tt2, npix));
+    t->flags |= MCFL_F_SQ;
+    t->bytes += 2 * npix;
+  }
+  if ((tallies & MCFL_TALLY_TEMP) && m.sim.do_temp_sim == 1 && !emission) {
+    MCFL_STCHK(ensure(t->time, ntime_of(&m.sim) * 8));
+    t->flags |= MCFL_F_TEMP;
+    t->bytes += ntime_of(&m.sim) * 8;
+  }
+  return MCFL_OK;
+}
+
+// Queue clearing of every tally buffer with the fill kernel (graph safe).
+mcfl_status clear_tallies(const Medium& m, Tallies* t, hipStream_t st) {
+  const uint64_t npix = m.npix, nvox = m.nvox;
+  auto fill = [&](DevBuf& b, uint64_t n) { return b.p ? mcfl_fill_u64(b.as<unsigned long long>(), n, st) : 0; };
+  if (fill(t->Rd, npix) || fill(t->Tt, npix) || fill(t->tot, MCFL_NTOT) || fill(t->fhd, nvox) ||
+      fill(t->axyz, nvox) || fill(t->Rd2, npix) || fill(t->Tt2, npix) || fill(t->time, ntime_of(&m.sim)))
+    return MCFL_ERR_DEVICE;
+  return MCFL_OK;
+}
+
+// Emission sources of a medium given its normalised fhd: weights
+// eY (1 - albedof) Fx over voxels whose material has albedof < 1
+// (CUDAMCFLmain.cu:550-556); returns the sum.
+mcfl_status build_emission(const Medium& m, const double* Fx, uint64_t photons, Tallies* t) {
+  std::vector<double> w;
+  std::vector<uint32_t> vox;
+  w.reserve(m.nvox);
+  vox.reserve(m.nvox);
+  double C = 0.0;
+  for (size_t v = 0; v < m.nvox; ++v) {
+    const mcfl_bulk& b = m.bulks[m.labels[v]];
+    if (!(b.albedof < 1)) continue;
+    const double c = (double)b.eY * (double)(1 - b.albedof) * Fx[v];
+    if (!(c > 0.0)) continue;
+    w.push_back(c);
+    vox.push_back((uint32_t)v);
+    C += c;
+  }
+  t->al_n = (uint32_t)vox.size();
+  t->fl_unit = photons ? C / ((double)photons * 4294967295.0) : 0.0;
+  if (vox.empty()) return MCFL_OK;  // nothing emits: the job launches no photon
+  std::vector<uint32_t> prob, alias;
+  build_alias(w, prob, alias);
+  const size_t n = vox.size() * 4;
+  MCFL_STCHK(ensure(t->al_prob, n));
+  MCFL_STCHK(ensure(t->al_alias, n));
+  MCFL_STCHK(ensure(t->al_vox, n));
+  MCFL_HIPCHK(hipMemcpy(t->al_prob.p, prob.data(), n, hipMemcpyHostToDevice));
+  MCFL_HIPCHK(hipMemcpy(t->al_alias.p, alias.data(), n, hipMemcpyHostToDevice));
+  MCFL_HIPCHK(hipMemcpy(t->al_vox.p, vox.data(), n, hipMemcpyHostToDevice));
+  t->bytes += 3 * n;
+  return MCFL_OK;
+}
+
+int src_kind_dev(int32_t k) {
+  switch (k) {
+    case MCFL_SOURCE_ISOTROPIC: return MCFL_SRC_ISOTROPIC;
+    case MCFL_SOURCE_DISK: return MCFL_SRC_DISK;
+    case MCFL_SOURCE_EMISSION: return MCFL_SRC_EMISSION;
+    default: return MCFL_SRC_PENCIL;
+  }
+}
+
+void fill_job(JobDev& J, const Medium& m, Tallies& t, int src_kind, float xi, float yi, float zi, float src_r,
+              uint64_t seed, uint64_t begin, uint64_t count, uint64_t k0, uint32_t mat_base) {
+  memset(&J, 0, sizeof J);
+  const bool em = src_kind == MCFL_SOURCE_EMISSION;
+  J.begin = begin;
+  J.count = count;
+  J.key = mcfl_key(seed, em ? 1u : 0u);
+  J.k0 = k0;
+  J.src_kind = src_kind_dev(src_kind);
+  J.xi = xi;
+  J.yi = yi;
+  J.zi = zi;
+  J.src_r = src_r;
+  J.start_weight = m.sim.start_weight;
+  J.mat_base = mat_base;
+  J.flags = t.flags;
+  J.Rd = t.Rd.as<unsigned long long>();
+  J.Tt = t.Tt.as<unsigned long long>();
+  J.fhd = t.fhd.as<unsigned long long>();
+  J.axyz = t.axyz.as<unsigned long long>();
+  J.time = t.time.as<unsigned long long>();
+  J.Rd2 = t.Rd2.as<unsigned long long>();
+  J.Tt2 = t.Tt2.as<unsigned long long>();
+  J.tot = t.tot.as<unsigned long long>();
+  fill_det(&m.sim, &J.det, m.d_tpos.as<const float>());
+  fill_medium(&m.sim, m.d_labels.as<const uint8_t>(), &J.med);
+  J.al_prob = t.al_prob.as<const uint32_t>();
+  J.al_alias = t.al_alias.as<const uint32_t>();
+  J.al_vox = t.al_vox.as<const uint32_t>();
+  J.al_n = t.al_n;
+}
+
+// Concatenated material tables of the (medium, emission) pairs a launch uses.
+struct MatTable {
+  std::vector<MatDev> mats;
+  std::vector<std::pair<const Medium*, bool>> keys;
+  std::vector<uint32_t> bases;
+  uint32_t base_of(const Medium* m, bool em) {
+    for (size_t i = 0; i < keys.size(); ++i)
+      if (keys[i].first == m && keys[i].second == em) return bases[i];
+    const std::vector<MatDev>& src = em ? m->mat_em : m->mat_ex;
+    uint32_t b = (uint32_t)mats.size();
+    mats.insert(mats.end(), src.begin(), src.end());
+    keys.emplace_back(m, em);
+    bases.push_back(b);
+    return b;
+  }
+};
+
+// Download one job's tallies into caller buffers.
+mcfl_status download(const Medium& m, const Tallies& t, uint64_t* Rd, uint64_t* Tt, uint64_t* fhd, uint64_t* axyz,
+                     uint64_t* Rd2, uint64_t* Tt2, uint64_t* tim, uint64_t* totals) {
+  const size_t npix = m.npix * 8, nvox = m.nvox * 8;
+  if (Rd) MCFL_HIPCHK(hipMemcpy(Rd, t.Rd.p, npix, hipMemcpyDeviceToHost));
+  if (Tt) MCFL_HIPCHK(hipMemcpy(Tt, t.Tt.p, npix, hipMemcpyDeviceToHost));
+  if (fhd && t.fhd.p) MCFL_HIPCHK(hipMemcpy(fhd, t.fhd.p, nvox, hipMemcpyDeviceToHost));
+  if (axyz && t.axyz.p) MCFL_HIPCHK(hipMemcpy(axyz, t.axyz.p, nvox, hipMemcpyDeviceToHost));
+  if (Rd2 && t.Rd2.p) MCFL_HIPCHK(hipMemcpy(Rd2, t.Rd2.p, npix, hipMemcpyDeviceToHost));
+  if (Tt2 && t.Tt2.p) MCFL_HIPCHK(hipMemcpy(Tt2, t.Tt2.p, npix, hipMemcpyDeviceToHost));
+  if (tim && t.time.p) MCFL_HIPCHK(hipMemcpy(tim, t.time.p, ntime_of(&m.sim) * 8, hipMemcpyDeviceToHost));
+  if (totals) MCFL_HIPCHK(hipMemcpy(totals, t.tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost));
+  return MCFL_OK;
+}
+
+// Shared launch plumbing: job table + material table upload, args.
+struct Launcher {
+  DevBuf d_jobs, d_mats, d_counter;
+  ~Launcher() {
+    release(d_jobs);
+    release(d_mats);
+    release(d_counter);
+  }
+  mcfl_status prepare(mcfl_ctx* c, const std::vector<JobDev>& jd, const MatTable& mt, bool perjob,
+                      const Medium* uniform, LaunchArgs* A) {
+    if (mt.mats.size() > MCFL_MAX_MATERIALS) return MCFL_ERR_CAPACITY;
+    if (jd.size() > MCFL_MAX_JOBS) return MCFL_ERR_CAPACITY;
+    MCFL_STCHK(ensure(d_jobs, jd.size() * sizeof(JobDev)));
+    MCFL_STCHK(ensure(d_mats, mt.mats.size() * sizeof(MatDev)));
+    MCFL_STCHK(ensure(d_counter, 8));
+    MCFL_HIPCHK(hipMemcpyAsync(d_jobs.p, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, c->stream));
+    MCFL_HIPCHK(hipMemcpyAsync(d_mats.p, mt.mats.data(), mt.mats.size() * sizeof(MatDev), hipMemcpyHostToDevice,
+                               c->stream));
+    memset(A, 0, sizeof *A);
+    if (!perjob) fill_medium(&uniform->sim, uniform->d_labels.as<const uint8_t>(), &A->med);
+    A->mats = d_mats.as<const MatDev>();
+    A->n_mats = (uint32_t)mt.mats.size();
+    A->work = MCFL_WORK_JOBS;
+    uint64_t total = 0;
+    for (const JobDev& J : jd) total += J.count;
+    A->total = total;
+    A->counter = d_counter.as<unsigned long long>();
+    A->a_tab = c->d_atab;
+    A->a_mask = kTablePhotons - 1;
+    A->jobs = d_jobs.as<const JobDev>();
+    A->njobs = (uint32_t)jd.size();
+    A->chunk_max = 1024;
+    return MCFL_OK;
+  }
+};
+
+}  // namespace
+
+// ================================================================== batch
+
+struct mcfl_batch {
+  mcfl_ctx* c = nullptr;
+  std::vector<std::unique_ptr<Medium>> media;
+  std::vector<std::unique_ptr<Tallies>> tal;
+  std::vector<mcfl_job> spec;
+  std::vector<JobDev> jd;
+  MatTable mt;
+  Launcher L;
+  LaunchArgs A{};
+  int temporal = 0, perjob = 0, nblocks = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* media, int n_media, const mcfl_job* jobs,
+                                         int n_jobs, mcfl_batch** out) {
+  if (!c || !media || n_media <= 0 || !jobs || n_jobs <= 0 || !out) return MCFL_ERR_ARG;
+  if (n_jobs > MCFL_MAX_BATCH_JOBS) return MCFL_ERR_CAPACITY;
+  *out = nullptr;
+  if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
+  std::unique_ptr<mcfl_batch> b(new (std::nothrow) mcfl_batch());
+  if (!b) return MCFL_ERR_NOMEM;
+  b->c = c;
+  for (int i = 0; i < n_media; ++i) {
+    b->media.emplace_back(new Medium());
+    MCFL_STCHK(medium_make(media[i], b->media.back().get()));
+  }
+  uint64_t begin = 0;
+  std::vector<bool> used(n_media, false);
+  for (int j = 0; j < n_jobs; ++j) {
+    const mcfl_job& s = jobs[j];
+    if (s.medium < 0 || s.medium >= n_media || s.photons == 0) return MCFL_ERR_ARG;
+    if (s.src_kind == MCFL_SOURCE_EMISSION && !s.Fx) return MCFL_ERR_ARG;
+    const Medium& m = *b->media[s.medium];
+    used[s.medium] = true;
+    b->tal.emplace_back(new Tallies());
+    Tallies& t = *b->tal.back();
+    const bool em = s.src_kind == MCFL_SOURCE_EMISSION;
+    MCFL_STCHK(alloc_tallies(m, s.tallies, em, &t));
+    if (em) MCFL_STCHK(build_emission(m, s.Fx, s.photons, &t));
+    if (t.flags & MCFL_F_TEMP) b->temporal = 1;
+    const uint64_t count = (em && t.al_n == 0) ? 0 : s.photons;
+    JobDev J;
+    fill_job(J, m, t, s.src_kind, s.xi, s.yi, s.zi, s.src_r, s.seed, begin, count, 0, b->mt.base_of(&m, em));
+    b->jd.push_back(J);
+    b->spec.push_back(s);
+    begin += count;
+  }
+  int n_used = 0;
+  const Medium* uni = nullptr;
+  for (int i = 0; i < n_media; ++i)
+    if (used[i]) {
+      ++n_used;
+      uni = b->media[i].get();
+    }
+  b->perjob = n_used > 1;
+  MCFL_STCHK(b->L.prepare(c, b->jd, b->mt, b->perjob, uni, &b->A));
+  b->nblocks = grid_blocks(c, MCFL_WORK_JOBS, b->temporal, b->perjob);
+  b->A.n_waves = (uint32_t)b->nblocks * 4u;
+  MCFL_HIPCHK(hipStreamSynchronize(c->stream));
+
+  // Capture: clear every tally and the work counter, then the transport launch.
+  MCFL_HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  mcfl_status st = MCFL_OK;
+  for (int j = 0; j < n_jobs && st == MCFL_OK; ++j) st = clear_tallies(*b->media[jobs[j].medium], b->tal[j].get(), c->stream);
+  if (st == MCFL_OK && mcfl_fill_u64(b->A.counter, 1, c->stream) != 0) st = MCFL_ERR_DEVICE;
+  if (st == MCFL_OK && b->A.total > 0 &&
+      mcfl_transport_launch(&b->A, b->temporal, b->perjob, b->nblocks, c->stream) != 0)
+    st = MCFL_ERR_DEVICE;
+  hipError_t e = hipStreamEndCapture(c->stream, &b->graph);
+  if (st != MCFL_OK) return st;
+  if (e != hipSuccess) return MCFL_ERR_DEVICE;
+  MCFL_HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  *out = b.release();
+  return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_batch_run(mcfl_batch* b, mcfl_job* jobs, double* kernel_ms) {
+  if (!b || !jobs) return MCFL_ERR_ARG;
+  mcfl_ctx* c = b->c;
+  if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
+  bool changed = false;
+  for (size_t j = 0; j < b->jd.size(); ++j) {
+    const uint64_t key = mcfl_key(jobs[j].seed, b->spec[j].src_kind == MCFL_SOURCE_EMISSION ? 1u : 0u);
+    if (key != b->jd[j].key) {
+      b->jd[j].key = key;
+      changed = true;
+    }
+  }
+  if (changed)
+    MCFL_HIPCHK(hipMemcpyAsync(b->L.d_jobs.p, b->jd.data(), b->jd.size() * sizeof(JobDev), hipMemcpyHostToDevice,
+                               c->stream));
+  MCFL_HIPCHK(hipEventRecord(c->ev0, c->stream));
+  MCFL_HIPCHK(hipGraphLaunch(b->exec, c->stream));
+  MCFL_HIPCHK(hipEventRecord(c->ev1, c->stream));
+  MCFL_HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  MCFL_HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (kernel_ms) *kernel_ms = ms;
+  for (size_t j = 0; j < b->jd.size(); ++j) {
+    mcfl_job& o = jobs[j];
+    const Medium& m = *b->media[b->spec[j].medium];
+    MCFL_STCHK(download(m, *b->tal[j], o.Rd_xy, o.Tt_xy, o.fhd, o.A_xyz, o.Rd2_xy, o.Tt2_xy, o.time_xyt, o.totals));
+    o.fl_unit = b->tal[j]->fl_unit;
+  }
+  return MCFL_OK;
+}
+
+extern "C" void mcfl_batch_destroy(mcfl_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->c->device);
+  (void)hipStreamSynchronize(b->c->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  delete b;
+}
+
+extern "C" mcfl_status mcfl_run_batch(mcfl_ctx* c, const mcfl_sim* const* media, int n_media, mcfl_job* jobs,
+                                      int n_jobs, double* kernel_ms) {
+  mcfl_batch* b = nullptr;
+  MCFL_STCHK(mcfl_batch_create(c, media, n_media, jobs, n_jobs, &b));
+  mcfl_status st = mcfl_batch_run(b, jobs, kernel_ms);
+  mcfl_batch_destroy(b);
+  return st;
+}
+
+// ================================================================= server
+
+namespace {
+
+struct Request {
+  int64_t id = 0;
+  mcfl_request r{};
+  int phase = 0;  // 0 queued, 1 excitation, 2 emission, 3 done
+  uint64_t done = 0;
+  uint64_t epochs = 0;
+  size_t footprint = 0;
+  Medium* m = nullptr;
+  std::unique_ptr<Tallies> ex, em;
+  std::vector<uint64_t> Rd, Tt, FlR, FlT;
+  uint64_t tot[MCFL_NTOT] = {}, fl_tot[MCFL_NTOT] = {};
+  double fl_unit = 0.0;
+};
+
+size_t footprint_of(const Medium& m, const mcfl_request& r) {
+  size_t b = 2 * m.npix * 8 + MCFL_NTOT * 8;
+  if (r.fluorescence) b += m.nvox * 8 + 2 * m.npix * 8 + MCFL_NTOT * 8 + 3 * 4 * m.nvox;
+  return b;
+}
+
+}  // namespace
+
+struct mcfl_server {
+  mcfl_ctx* c = nullptr;
+  uint64_t budget = 0, epoch_photons = 0, in_use = 0;
+  std::vector<std::unique_ptr<Medium>> media;
+  std::deque<std::unique_ptr<Request>> queue;
+  std::vector<std::unique_ptr<Request>> active;
+  std::map<int64_t, std::unique_ptr<Request>> done;
+  int64_t next_id = 1;
+  Launcher L;
+};
+
+extern "C" mcfl_status mcfl_server_create(mcfl_ctx* c, uint64_t budget, uint64_t epoch_photons, mcfl_server** out) {
+  if (!c || !out || epoch_photons == 0) return MCFL_ERR_ARG;
+  mcfl_server* s = new (std::nothrow) mcfl_server();
+  if (!s) return MCFL_ERR_NOMEM;
+  s->c = c;
+  s->budget = budget;
+  s->epoch_photons = epoch_photons;
+  *out = s;
+  return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_server_add_medium(mcfl_server* s, const mcfl_sim* sim, int32_t* id) {
+  if (!s || !sim || !id) return MCFL_ERR_ARG;
+  if (hipSetDevice(s->c->device) != hipSuccess) return MCFL_ERR_DEVICE;
+  std::unique_ptr<Medium> m(new (std::nothrow) Medium());
+  if (!m) return MCFL_ERR_NOMEM;
+  MCFL_STCHK(medium_make(sim, m.get()));
+  if (m->mat_ex.size() > MCFL_MAX_MATERIALS / 2) return MCFL_ERR_CAPACITY;
+  s->media.push_back(std::move(m));
+  *id = (int32_t)s->media.size() - 1;
+  return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_server_submit(mcfl_server* s, const mcfl_request* r, int64_t* id) {
+  if (!s || !r || !id || r->photons == 0 || r->medium < 0 || r->medium >= (int32_t)s->media.size())
+    return MCFL_ERR_ARG;
+  if (r->src_kind == MCFL_SOURCE_EMISSION) return MCFL_ERR_ARG;
+  std::unique_ptr<Request> q(new (std::nothrow) Request());
+  if (!q) return MCFL_ERR_NOMEM;
+  q->id = s->next_id++;
+  q->r = *r;
+  q->m = s->media[r->medium].get();
+  q->footprint = footprint_of(*q->m, *r);
+  if (q->footprint > s->budget) return MCFL_ERR_CAPACITY;
+  *id = q->id;
+  s->queue.push_back(std::move(q));
+  return MCFL_OK;
+}
+
+namespace {
+
+mcfl_status admit(mcfl_server* s) {
+  while (!s->queue.empty() && s->active.size() < MCFL_MAX_JOBS) {
+    Request& q = *s->queue.front();
+    if (s->in_use + q.footprint > s->budget) break;  // admission by memory footprint, FIFO
+    q.ex.reset(new Tallies());
+    MCFL_STCHK(alloc_tallies(*q.m, q.r.fluorescence ? (uint32_t)MCFL_TALLY_FHD : 0u, false, q.ex.get()));
+    MCFL_STCHK(clear_tallies(*q.m, q.ex.get(), s->c->stream));
+    if (q.r.fluorescence) {
+      q.em.reset(new Tallies());
+      MCFL_STCHK(alloc_tallies(*q.m, 0u, true, q.em.get()));
+      MCFL_STCHK(clear_tallies(*q.m, q.em.get(), s->c->stream));
+    }
+    q.phase = 1;
+    s->in_use += q.footprint;
+    s->active.push_back(std::move(s->queue.front()));
+    s->queue.pop_front();
+  }
+  return MCFL_OK;
+}
+
+// Phase finished: start the emission pass or retire the request.
+mcfl_status advance(mcfl_server* s, Request& q) {
+  const Medium& m = *q.m;
+  if (q.phase == 1) {
+    q.Rd.resize(m.npix);
+    q.Tt.resize(m.npix);
+    MCFL_STCHK(download(m, *q.ex, q.Rd.data(), q.Tt.data(), nullptr, nullptr, nullptr, nullptr, nullptr, q.tot));
+    if (q.r.fluorescence) {
+      std::vector<uint64_t> fhd(m.nvox);
+      MCFL_HIPCHK(hipMemcpy(fhd.data(), q.ex->fhd.p, m.nvox * 8, hipMemcpyDeviceToHost));
+      std::vector<double> Fx(m.nvox);
+      const double norm = 4294967295.0 * (double)q.tot[MCFL_TOT_PHOTONS];  // CUDAMCFLmain.cu:181-183
+      for (size_t v = 0; v < m.nvox; ++v) Fx[v] = (double)fhd[v] / norm;
+      MCFL_STCHK(build_emission(m, Fx.data(), q.r.photons, q.em.get()));
+      q.fl_unit = q.em->fl_unit;
+      q.phase = 2;
+      q.done = 0;
+      if (q.em->al_n > 0) return MCFL_OK;
+    }
+  }
+  if (q.phase == 2) {
+    q.FlR.resize(m.npix);
+    q.FlT.resize(m.npix);
+    MCFL_STCHK(download(m, *q.em, q.FlR.data(), q.FlT.data(), nullptr, nullptr, nullptr, nullptr, nullptr, q.fl_tot));
+  }
+  q.phase = 3;
+  q.ex.reset();
+  q.em.reset();
+  s->in_use -= q.footprint;
+  return MCFL_OK;
+}
+
+}  // namespace
+
+extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32_t max_done, int32_t* n_done,
+                                        double* kernel_ms) {
+  if (!s) return MCFL_ERR_ARG;
+  mcfl_ctx* c = s->c;
+  if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
+  if (n_done) *n_done = 0;
+  if (kernel_ms) *kernel_ms = 0.0;
+  MCFL_STCHK(admit(s));
+  if (s->active.empty()) return MCFL_OK;
+
+  // Water-filling share of the epoch over the active requests.
+  const size_t n = s->active.size();
+  std::vector<uint64_t> rem(n), take(n, 0);
+  for (size_t i = 0; i < n; ++i) rem[i] = s->active[i]->r.photons - s->active[i]->done;
+  uint64_t left = s->epoch_photons;
+  size_t open = n;
+  while (left > 0 && open > 0) {
+    const uint64_t share = std::max<uint64_t>(1, left / open);
+    open = 0;
+    for (size_t i = 0; i < n && left > 0; ++i) {
+      const uint64_t want = rem[i] - take[i];
+      if (want == 0) continue;
+      const uint64_t t = std::min({want, share, left});
+      take[i] += t;
+      left -= t;
+      if (take[i] < rem[i]) ++open;
+    }
+  }
+
+  std::vector<JobDev> jd;
+  MatTable mt;
+  const Medium* uni = s->active[0]->m;
+  bool perjob = false;
+  uint64_t begin = 0;
+  for (size_t i = 0; i < n; ++i) {
+    Request& q = *s->active[i];
+    if (q.m != uni) perjob = true;
+    const bool em = q.phase == 2;
+    Tallies& t = em ? *q.em : *q.ex;
+    JobDev J;
+    fill_job(J, *q.m, t, em ? MCFL_SOURCE_EMISSION : q.r.src_kind, q.r.xi, q.r.yi, q.r.zi, q.r.src_r, q.r.seed,
+             begin, take[i], q.done, mt.base_of(q.m, em));
+    jd.push_back(J);
+    begin += take[i];
+    if (take[i] > 0) ++q.epochs;
+  }
+  LaunchArgs A;
+  MCFL_STCHK(s->L.prepare(c, jd, mt, perjob, uni, &A));
+  MCFL_HIPCHK(hipMemsetAsync(A.counter, 0, 8, c->stream));
+  double ms = 0.0;
+  if (A.total > 0) MCFL_STCHK(launch_and_time(c, A, 0, perjob ? 1 : 0, &ms));
+  MCFL_HIPCHK(hipStreamSynchronize(c->stream));
+  if (kernel_ms) *kernel_ms = ms;
+
+  int32_t nd = 0;
+  std::vector<std::unique_ptr<Request>> still;
+  for (size_t i = 0; i < n; ++i) {
+    std::unique_ptr<Request>& q = s->active[i];
+    q->done += take[i];
+    if (q->done >= q->r.photons) MCFL_STCHK(advance(s, *q));
+    if (q->phase == 3) {
+      if (done_ids && nd < max_done) done_ids[nd] = q->id;
+      ++nd;
+      s->done[q->id] = std::move(q);
+    } else {
+      still.push_back(std::move(q));
+    }
+  }
+  s->active.swap(still);
+  if (n_done) *n_done = nd;
+  return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_server_collect(mcfl_server* s, int64_t id, mcfl_request_result* out) {
+  if (!s || !out) return MCFL_ERR_ARG;
+  auto it = s->done.find(id);
+  if (it == s->done.end()) return MCFL_ERR_ARG;
+  Request& q = *it->second;
+  const size_t npix = q.m->npix * 8;
+  if (out->Rd_xy) memcpy(out->Rd_xy, q.Rd.data(), npix);
+  if (out->Tt_xy) memcpy(out->Tt_xy, q.Tt.data(), npix);
+  if (out->FlR_raw && !q.FlR.empty()) memcpy(out->FlR_raw, q.FlR.data(), npix);
+  if (out->FlT_raw && !q.FlT.empty()) memcpy(out->FlT_raw, q.FlT.data(), npix);
+  out->fl_unit = q.fl_unit;
+  memcpy(out->totals, q.tot, sizeof q.tot);
+  memcpy(out->fl_totals, q.fl_tot, sizeof q.fl_tot);
+  out->epochs = q.epochs;
+  s->done.erase(it);
+  return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_server_stats(mcfl_server* s, int32_t* queued, int32_t* active, uint64_t* bytes) {
+  if (!s) return MCFL_ERR_ARG;
+  if (queued) *queued = (int32_t)s->queue.size();
+  if (active) *active = (int32_t)s->active.size();
+  if (bytes) *bytes = s->in_use;
+  return MCFL_OK;
+}
+
+extern "C" void mcfl_server_destroy(mcfl_server* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->c->device);
+  (void)hipStreamSynchronize(s->c->stream);
+  delete s;
+}

@@ -14,42 +14,17 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <new>
 #include <string>
 #include <vector>
 
-#include <hip/hip_runtime_api.h>
-
-#include "../../include/mcfl.h"
-#include "mcfl_device.h"
+#include "mcfl_internal.h"
 #include "mcfl_mwc.h"
 
-extern "C" int mcfl_transport_occupancy(int work, int temporal, int* blocks_per_cu);
-extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int nblocks, hipStream_t stream);
+#define HIPCHK MCFL_HIPCHK
+#define STCHK MCFL_STCHK
 
-namespace {
-
-constexpr uint32_t kTablePhotons = 32768;  // photon-stream multipliers: lines 2..32769 of the primes file
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-};
-
-}  // namespace
-
-struct mcfl_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int n_cu = 0;
-  int bpc[3][2] = {};
-  char name[256] = {};
-  uint32_t* d_atab = nullptr;
-  std::vector<uint32_t> atab;
-  DevBuf labels, mats, jobs, counter, tot, Rd, Tt, fhd, axyz, time, Rd2, Tt2, tpos, xs, as, vox, vscale;
-};
-
-namespace {
+namespace mcfl_detail {
 
 mcfl_status ensure(DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 8;
@@ -66,49 +41,6 @@ void release(DevBuf& b) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.cap = 0;
-}
-
-#define HIPCHK(expr)                                 \
-  do {                                               \
-    if ((expr) != hipSuccess) return MCFL_ERR_DEVICE; \
-  } while (0)
-#define STCHK(expr)                        \
-  do {                                     \
-    mcfl_status st_ = (expr);              \
-    if (st_ != MCFL_OK) return st_;        \
-  } while (0)
-
-std::string lib_dir() {
-  Dl_info info;
-  if (dladdr(reinterpret_cast<void*>(&mcfl_default_primes_path), &info) && info.dli_fname) {
-    std::string p(info.dli_fname);
-    size_t s = p.rfind('/');
-    return s == std::string::npos ? std::string(".") : p.substr(0, s);
-  }
-  return ".";
-}
-
-// Reads the first column of n lines after the seeding line.
-bool read_multipliers(const char* path, uint32_t n, std::vector<uint32_t>& out, uint32_t* begin) {
-  FILE* f = fopen(path, "r");
-  if (!f) return false;
-  unsigned a0, t1, t2;
-  if (fscanf(f, "%u %u %u", &a0, &t1, &t2) != 3) {
-    fclose(f);
-    return false;
-  }
-  if (begin) *begin = a0;
-  out.resize(n);
-  for (uint32_t i = 0; i < n; ++i) {
-    unsigned a;
-    if (fscanf(f, "%u %u %u", &a, &t1, &t2) != 3) {
-      fclose(f);
-      return false;
-    }
-    out[i] = a;
-  }
-  fclose(f);
-  return true;
 }
 
 // Per-material constants of the kernel (mcfl_device.h MatDev).  emission:
@@ -135,13 +67,16 @@ void build_mats(const mcfl_sim* s, bool emission, std::vector<MatDev>& out) {
 
 size_t nvox_of(const mcfl_sim* s) { return (size_t)s->nx * (size_t)s->ny * (size_t)s->nz; }
 size_t npix_of(const mcfl_sim* s) { return (size_t)s->det.nx * (size_t)(s->det.ny + 1); }
+size_t ntime_of(const mcfl_sim* s) {
+  return (size_t)s->det.x_temp_numdets * (size_t)s->det.y_temp_numdets * (size_t)s->det.temp_bins;
+}
 
 mcfl_status validate(const mcfl_sim* s) {
   if (!s) return MCFL_ERR_ARG;
   if (s->bulk_method != 2) return MCFL_ERR_UNSUPPORTED;
   if (s->nx <= 0 || s->ny <= 0 || s->nz <= 0 || !s->bulk_info || !s->bulks) return MCFL_ERR_ARG;
   if (s->grid_size <= 0 || !(s->esp > 0.0f)) return MCFL_ERR_ARG;
-  if (s->n_bulks < 1 || s->n_bulks + 2 > MCFL_MAX_MATERIALS) return MCFL_ERR_CAPACITY;
+  if (s->n_bulks < 1 || s->n_bulks > 98) return MCFL_ERR_CAPACITY;
   if (s->det.nx <= 0 || s->det.ny <= 0) return MCFL_ERR_ARG;
   if (s->do_temp_sim && (s->det.temp_bins == 0 || s->det.max_temp == 0)) return MCFL_ERR_ARG;
   const size_t nv = nvox_of(s);
@@ -152,11 +87,8 @@ mcfl_status validate(const mcfl_sim* s) {
   return MCFL_OK;
 }
 
-mcfl_status medium_upload(mcfl_ctx* c, const mcfl_sim* s, MediumDev* md) {
-  const size_t nv = nvox_of(s);
-  STCHK(ensure(c->labels, nv));
-  HIPCHK(hipMemcpyAsync(c->labels.p, s->bulk_info, nv, hipMemcpyHostToDevice, c->stream));
-  md->labels = static_cast<const uint8_t*>(c->labels.p);
+void fill_medium(const mcfl_sim* s, const uint8_t* d_labels, MediumDev* md) {
+  md->labels = d_labels;
   md->nx = s->nx;
   md->ny = s->ny;
   md->nz = s->nz;
@@ -165,7 +97,6 @@ mcfl_status medium_upload(mcfl_ctx* c, const mcfl_sim* s, MediumDev* md) {
   md->gs = (float)s->grid_size;
   md->halfx = s->halfx;
   md->halfy = s->halfy;
-  return MCFL_OK;
 }
 
 void fill_det(const mcfl_sim* s, DetDev* d, const float* d_tpos) {
@@ -203,18 +134,68 @@ int src_kind_of(const mcfl_sim* s) {
   return s->src_kind == MCFL_SOURCE_DISK ? MCFL_SRC_DISK : MCFL_SRC_PENCIL;
 }
 
-mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, double* ms) {
-  const int bpc = c->bpc[A.work][temporal ? 1 : 0];
-  const int nblocks = c->n_cu * (bpc > 0 ? bpc : 1);
+int grid_blocks(const mcfl_ctx* c, int work, int temporal, int perjob) {
+  const int bpc = c->bpc[work][temporal ? 1 : 0][perjob ? 1 : 0];
+  return c->n_cu * (bpc > 0 ? bpc : 1);
+}
+
+mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms) {
+  const int nblocks = grid_blocks(c, A.work, temporal, perjob);
   A.n_waves = (uint32_t)nblocks * 4u;
-  A.chunk_max = 1024;
+  if (A.chunk_max == 0) A.chunk_max = 1024;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (mcfl_transport_launch(&A, temporal, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
+  if (mcfl_transport_launch(&A, temporal, perjob, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));
   float t = 0.f;
   HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
   if (ms) *ms = t;
+  return MCFL_OK;
+}
+
+}  // namespace mcfl_detail
+
+using namespace mcfl_detail;
+
+namespace {
+
+std::string lib_dir() {
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&mcfl_default_primes_path), &info) && info.dli_fname) {
+    std::string p(info.dli_fname);
+    size_t s = p.rfind('/');
+    return s == std::string::npos ? std::string(".") : p.substr(0, s);
+  }
+  return ".";
+}
+
+// Reads the first column of n lines after the seeding line.
+bool read_multipliers(const char* path, uint32_t n, std::vector<uint32_t>& out) {
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  unsigned a0, t1, t2;
+  if (fscanf(f, "%u %u %u", &a0, &t1, &t2) != 3) {
+    fclose(f);
+    return false;
+  }
+  out.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    unsigned a;
+    if (fscanf(f, "%u %u %u", &a, &t1, &t2) != 3) {
+      fclose(f);
+      return false;
+    }
+    out[i] = a;
+  }
+  fclose(f);
+  return true;
+}
+
+mcfl_status medium_upload(mcfl_ctx* c, const mcfl_sim* s, MediumDev* md) {
+  const size_t nv = nvox_of(s);
+  STCHK(ensure(c->labels, nv));
+  HIPCHK(hipMemcpyAsync(c->labels.p, s->bulk_info, nv, hipMemcpyHostToDevice, c->stream));
+  fill_medium(s, c->labels.as<const uint8_t>(), md);
   return MCFL_OK;
 }
 
@@ -306,11 +287,12 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
   for (int w = 0; w < 3; ++w)
     for (int t = 0; t < 2; ++t)
-      if (mcfl_transport_occupancy(w, t, &c->bpc[w][t]) != 0) return fail(MCFL_ERR_DEVICE);
+      for (int pj = 0; pj < 2; ++pj)
+        if (mcfl_transport_occupancy(w, t, pj, &c->bpc[w][t][pj]) != 0) return fail(MCFL_ERR_DEVICE);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   if (!primes || !primes[0]) primes = mcfl_default_primes_path();
-  if (!read_multipliers(primes, kTablePhotons, c->atab, nullptr)) return fail(MCFL_ERR_IO);
+  if (!read_multipliers(primes, kTablePhotons, c->atab)) return fail(MCFL_ERR_IO);
   if (hipMalloc(&c->d_atab, kTablePhotons * sizeof(uint32_t)) != hipSuccess) return fail(MCFL_ERR_NOMEM);
   if (hipMemcpy(c->d_atab, c->atab.data(), kTablePhotons * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
     return fail(MCFL_ERR_DEVICE);
@@ -335,7 +317,7 @@ extern "C" void mcfl_ctx_destroy(mcfl_ctx* c) {
 extern "C" mcfl_status mcfl_ctx_info(mcfl_ctx* c, int* n_cu, int* bpc, char* name, int name_len) {
   if (!c) return MCFL_ERR_ARG;
   if (n_cu) *n_cu = c->n_cu;
-  if (bpc) *bpc = c->bpc[MCFL_WORK_JOBS][0];
+  if (bpc) *bpc = c->bpc[MCFL_WORK_JOBS][0][0];
   if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->name);
   return MCFL_OK;
 }
@@ -449,7 +431,7 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     A.work = MCFL_WORK_JOBS;
     A.total = s->number_of_photons;
   }
-  STCHK(launch_and_time(c, A, temporal, &out->kernel_ms));
+  STCHK(launch_and_time(c, A, temporal, 0, &out->kernel_ms));
 
   HIPCHK(hipMemcpy(out->Rd_xy, c->Rd.p, npix * 8, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(out->Tt_xy, c->Tt.p, npix * 8, hipMemcpyDeviceToHost));
@@ -588,7 +570,7 @@ extern "C" mcfl_status mcfl_run_fluorescence(mcfl_ctx* c, const mcfl_sim* s, con
   A.per_job = nper;
   A.vseed = seed;
   double ms = 0.0;
-  if (A.total > 0) STCHK(launch_and_time(c, A, 0, &ms));
+  if (A.total > 0) STCHK(launch_and_time(c, A, 0, 0, &ms));
 
   std::vector<uint64_t> R(npix), T(npix);
   HIPCHK(hipMemcpy(R.data(), c->Rd.p, npix * 8, hipMemcpyDeviceToHost));

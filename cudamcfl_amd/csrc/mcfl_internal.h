@@ -1,0 +1,75 @@
+// mcfl_internal.h -- host-side internals shared by the single-run entry
+// points (mcfl_host.cpp) and the batch / serving engine (mcfl_engine.cpp).
+#ifndef MCFL_INTERNAL_H
+#define MCFL_INTERNAL_H
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/mcfl.h"
+#include "mcfl_device.h"
+
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int* blocks_per_cu);
+extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream);
+extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
+
+namespace mcfl_detail {
+
+constexpr uint32_t kTablePhotons = 32768;  // photon-stream multipliers: lines 2..32769 of the primes file
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+mcfl_status ensure(DevBuf& b, size_t bytes);
+void release(DevBuf& b);
+
+void build_mats(const mcfl_sim* s, bool emission, std::vector<MatDev>& out);
+size_t nvox_of(const mcfl_sim* s);
+size_t npix_of(const mcfl_sim* s);
+size_t ntime_of(const mcfl_sim* s);
+mcfl_status validate(const mcfl_sim* s);
+void fill_medium(const mcfl_sim* s, const uint8_t* d_labels, MediumDev* md);
+void fill_det(const mcfl_sim* s, DetDev* d, const float* d_tpos);
+void tdet_positions(const mcfl_sim* s, std::vector<float>& pos);
+int src_kind_of(const mcfl_sim* s);
+
+}  // namespace mcfl_detail
+
+struct mcfl_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int n_cu = 0;
+  int bpc[3][2][2] = {};  // resident workgroups per CU [work][temporal][perjob]
+  char name[256] = {};
+  uint32_t* d_atab = nullptr;
+  std::vector<uint32_t> atab;
+  mcfl_detail::DevBuf labels, mats, jobs, counter, tot, Rd, Tt, fhd, axyz, time, Rd2, Tt2, tpos, xs, as, vox, vscale;
+};
+
+namespace mcfl_detail {
+// Launch geometry, launch, HIP-event timing on the context stream.
+mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms);
+int grid_blocks(const mcfl_ctx* c, int work, int temporal, int perjob);
+}  // namespace mcfl_detail
+
+#define MCFL_HIPCHK(expr)                           \
+  do {                                              \
+    if ((expr) != hipSuccess) return MCFL_ERR_DEVICE; \
+  } while (0)
+#define MCFL_STCHK(expr)            \
+  do {                              \
+    mcfl_status st_ = (expr);       \
+    if (st_ != MCFL_OK) return st_; \
+  } while (0)
+
+#endif

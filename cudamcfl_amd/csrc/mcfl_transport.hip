@@ -253,7 +253,7 @@ __device__ void exit_tally(const JobDev& J, const Ph& p, bool top, bool scaled, 
         atomic_add_u64((top ? J.Rd2 : J.Tt2) + idx, ((unsigned long long)p.w * (unsigned long long)p.w) >> 24);
     }
   }
-  if (TEMPORAL && d.t_rort == (top ? 0u : 1u) && p.tof < d.t_max) {
+  if (TEMPORAL && (J.flags & MCFL_F_TEMP) && d.t_rort == (top ? 0u : 1u) && p.tof < d.t_max) {
     for (uint32_t xp = 0; xp < d.t_ndx; ++xp)
       for (uint32_t yp = 0; yp < d.t_ndy; ++yp) {
         const float ddx = p.x - d.t_posx[xp], ddy = p.y - d.t_posy[yp];
@@ -285,7 +285,7 @@ __device__ __forceinline__ uint32_t find_job(const LaunchArgs& A, uint64_t g, ui
 
 } /* namespace */
 
-template <int WORK, bool TEMPORAL>
+template <int WORK, bool TEMPORAL, bool PERJOB>
 __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
   __shared__ MatDev smat[MCFL_MAX_MATERIALS];
   __shared__ unsigned long long stot[MCFL_MAX_JOBS * MCFL_NTOT];
@@ -296,7 +296,8 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
   for (uint32_t i = tid; i < nslots * MCFL_NTOT; i += MCFL_WG) stot[i] = 0ull;
   __syncthreads();
 
-  const MediumDev& m = A.med;
+  /* the medium: launch-uniform, or the current photon's job's (PERJOB) */
+  MediumDev m = A.med;
   const uint32_t lane = __lane_id();
 
   Ph p = {};
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
         if (WORK == MCFL_WORK_JOBS) {
           job = find_job(A, g, job);
           const JobDev& J = A.jobs[job];
-          const uint64_t k = g - J.begin;
+          const uint64_t k = g - J.begin + J.k0;
           a = A.a_tab[k & A.a_mask];
           x = mcfl_photon_state(J.key, k, a);
           mbase = J.mat_base;
@@ -367,7 +368,20 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
           fhd = J.fhd;
           axyz = J.axyz;
           escale = J.exit_scale;
-          launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a);
+          if (PERJOB) m = J.med;
+          if (J.src_kind == MCFL_SRC_EMISSION) {
+            /* importance-sampled emission voxel (Walker alias table), then an
+               isotropic photon from its corner as in CUDAMCFLmain.cu:481-487 */
+            const uint32_t col = (uint32_t)(((uint64_t)mcfl_mwc_next(&x, a) * (uint64_t)J.al_n) >> 32);
+            const uint32_t u = mcfl_mwc_next(&x, a);
+            const uint32_t v = J.al_vox[u < J.al_prob[col] ? col : J.al_alias[col]];
+            const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
+            const uint32_t iz = v / nxy, rem = v - iz * nxy, iy = rem / (uint32_t)m.nx, ix = rem - iy * (uint32_t)m.nx;
+            launch(m, MCFL_SRC_ISOTROPIC, ((float)ix / m.gs) - m.halfx, ((float)iy / m.gs) - m.halfy,
+                   ((float)iz / m.gs), 0.0f, 0u, p, x, a);
+          } else {
+            launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a);
+          }
         } else if (WORK == MCFL_WORK_VOXELS) {
           const uint64_t j = g / A.per_job;
           const uint64_t k = g - j * A.per_job;
@@ -510,34 +524,46 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
   }
 }
 
+/* Zero-fill used inside captured graphs (no memset nodes in graphs). */
+__global__ void mcfl_fill_u64_kernel(unsigned long long* p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
 /* ------------------------------------------------------------------ host */
 
-extern "C" int mcfl_transport_occupancy(int work, int temporal, int* blocks_per_cu) {
-  hipError_t e;
-  const void* f;
-  if (work == MCFL_WORK_JOBS)
-    f = temporal ? (const void*)mcfl_transport_kernel<MCFL_WORK_JOBS, true>
-                 : (const void*)mcfl_transport_kernel<MCFL_WORK_JOBS, false>;
-  else if (work == MCFL_WORK_VOXELS)
-    f = temporal ? (const void*)mcfl_transport_kernel<MCFL_WORK_VOXELS, true>
-                 : (const void*)mcfl_transport_kernel<MCFL_WORK_VOXELS, false>;
-  else
-    f = temporal ? (const void*)mcfl_transport_kernel<MCFL_WORK_LANES, true>
-                 : (const void*)mcfl_transport_kernel<MCFL_WORK_LANES, false>;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, MCFL_WG, 0);
+namespace {
+template <int W, bool T, bool P>
+const void* kfn() {
+  return (const void*)mcfl_transport_kernel<W, T, P>;
+}
+const void* kernel_of(int work, int temporal, int perjob) {
+  if (work == MCFL_WORK_JOBS) {
+    if (perjob) return temporal ? kfn<MCFL_WORK_JOBS, true, true>() : kfn<MCFL_WORK_JOBS, false, true>();
+    return temporal ? kfn<MCFL_WORK_JOBS, true, false>() : kfn<MCFL_WORK_JOBS, false, false>();
+  }
+  if (work == MCFL_WORK_VOXELS) return temporal ? kfn<MCFL_WORK_VOXELS, true, false>() : kfn<MCFL_WORK_VOXELS, false, false>();
+  return temporal ? kfn<MCFL_WORK_LANES, true, false>() : kfn<MCFL_WORK_LANES, false, false>();
+}
+}  // namespace
+
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int* blocks_per_cu) {
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel_of(work, temporal, perjob), MCFL_WG, 0);
   return e == hipSuccess ? 0 : (int)e;
 }
 
-extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int nblocks, hipStream_t stream) {
-  dim3 grid((unsigned)nblocks), block(MCFL_WG);
-  switch (A->work * 2 + (temporal ? 1 : 0)) {
-    case MCFL_WORK_JOBS * 2 + 0: mcfl_transport_kernel<MCFL_WORK_JOBS, false><<<grid, block, 0, stream>>>(*A); break;
-    case MCFL_WORK_JOBS * 2 + 1: mcfl_transport_kernel<MCFL_WORK_JOBS, true><<<grid, block, 0, stream>>>(*A); break;
-    case MCFL_WORK_VOXELS * 2 + 0: mcfl_transport_kernel<MCFL_WORK_VOXELS, false><<<grid, block, 0, stream>>>(*A); break;
-    case MCFL_WORK_VOXELS * 2 + 1: mcfl_transport_kernel<MCFL_WORK_VOXELS, true><<<grid, block, 0, stream>>>(*A); break;
-    case MCFL_WORK_LANES * 2 + 0: mcfl_transport_kernel<MCFL_WORK_LANES, false><<<grid, block, 0, stream>>>(*A); break;
-    case MCFL_WORK_LANES * 2 + 1: mcfl_transport_kernel<MCFL_WORK_LANES, true><<<grid, block, 0, stream>>>(*A); break;
-    default: return -1;
-  }
+extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream) {
+  if (A->work < 0 || A->work > 2) return -1;
+  void* args[] = {const_cast<LaunchArgs*>(A)};
+  hipError_t e = hipLaunchKernel(kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0),
+                                 dim3((unsigned)nblocks), dim3(MCFL_WG), args, 0, stream);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream) {
+  if (n == 0) return 0;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  mcfl_fill_u64_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(p, n);
   return (int)hipGetLastError();
 }

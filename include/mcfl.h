@@ -65,7 +65,12 @@ typedef struct mcfl_det { /* DetStruct, CUDAMCFL.h:71-96 */
   uint64_t temp_bins, max_temp;
 } mcfl_det;
 
-enum { MCFL_SOURCE_ISOTROPIC = 0, MCFL_SOURCE_PENCIL = 1, MCFL_SOURCE_DISK = 2 };
+enum {
+  MCFL_SOURCE_ISOTROPIC = 0, /* isotropic point source at a voxel corner (the reference's emission source) */
+  MCFL_SOURCE_PENCIL = 1,    /* collimated Gaussian spot, w = 0.03 cm (the reference's excitation source) */
+  MCFL_SOURCE_DISK = 2,      /* collimated uniform disk of radius src_r (extension) */
+  MCFL_SOURCE_EMISSION = 3   /* importance-sampled fluorescence emission of a whole medium (extension) */
+};
 
 typedef struct mcfl_sim { /* SimulationStruct, CUDAMCFL.h:129-170 */
   uint64_t number_of_photons;
@@ -210,6 +215,92 @@ mcfl_status mcfl_fl_plan(const mcfl_sim* sim, const double* Fx, uint64_t n_per_v
  * may be NULL) receive the reference's images. */
 mcfl_status mcfl_run_fluorescence(mcfl_ctx* ctx, const mcfl_sim* sim, const double* Fx, uint64_t n_per_voxel,
                                   uint64_t seed, double* FlR, double* FlT, mcfl_fl_result* res);
+
+/* ---- batched launches ----
+ * No counterpart in the reference, whose multi-source scans run one process
+ * per .mci file (ScriptGenerator-CeCAR-Translation.ipynb:48-55,
+ * cudamcfl-array.sh:15).  Many simulations -- different sources, budgets and
+ * media -- share one persistent launch and one photon queue; each job's
+ * photons draw from streams keyed by its own seed, so every job's tallies are
+ * identical to a run of that job alone. */
+
+enum { MCFL_TALLY_FHD = 1, MCFL_TALLY_SQ = 2, MCFL_TALLY_TEMP = 4, MCFL_TALLY_AXYZ = 8 };
+#define MCFL_MAX_BATCH_JOBS 64
+
+typedef struct mcfl_job {
+  int32_t medium;          /* index into the media list */
+  int32_t src_kind;        /* MCFL_SOURCE_*; EMISSION needs Fx */
+  float xi, yi, zi, src_r; /* source position and disk radius [cm] */
+  uint64_t photons;        /* photon budget */
+  uint64_t seed;           /* photon-stream seed */
+  uint32_t tallies;        /* MCFL_TALLY_* */
+  const double* Fx;        /* EMISSION: normalised excitation fhd of the medium, nx*ny*nz */
+  /* outputs: caller-owned host buffers, Rd_xy/Tt_xy required, others as enabled */
+  uint64_t* Rd_xy;
+  uint64_t* Tt_xy;
+  uint64_t* fhd;
+  uint64_t* A_xyz;
+  uint64_t* Rd2_xy;
+  uint64_t* Tt2_xy;
+  uint64_t* time_xyt;
+  uint64_t totals[MCFL_NTOT];
+  double fl_unit; /* EMISSION: image value (times dx*dy) of one raw unit, sum(eY (1-albedof) Fx) / (photons (2^32-1)) */
+} mcfl_job;
+
+typedef struct mcfl_batch mcfl_batch;
+
+/* Uploads the media, allocates every job's device tallies and captures the
+ * batch (tally clearing + one transport launch) in a HIP graph. */
+mcfl_status mcfl_batch_create(mcfl_ctx* ctx, const mcfl_sim* const* media, int n_media, const mcfl_job* jobs,
+                              int n_jobs, mcfl_batch** out);
+/* Replays the graph with jobs[i].seed and copies each job's outputs into jobs[i]. */
+mcfl_status mcfl_batch_run(mcfl_batch* b, mcfl_job* jobs, double* kernel_ms);
+void mcfl_batch_destroy(mcfl_batch* b);
+/* create + run + destroy */
+mcfl_status mcfl_run_batch(mcfl_ctx* ctx, const mcfl_sim* const* media, int n_media, mcfl_job* jobs, int n_jobs,
+                           double* kernel_ms);
+
+/* ---- serving: continuous batching of forward-simulation requests ----
+ * Requests are admitted while their device footprint fits the memory budget
+ * and a job slot is free; each step runs one epoch kernel over a slice of
+ * every admitted request's photons, tallies stay resident in HBM between
+ * epochs, finished requests release their memory.  A fluorescence request
+ * runs its excitation, then an importance-sampled emission pass of the same
+ * budget.  A request's result does not depend on what it shared epochs with. */
+
+typedef struct mcfl_server mcfl_server;
+
+typedef struct mcfl_request {
+  int32_t medium; /* id returned by mcfl_server_add_medium */
+  int32_t src_kind;
+  float xi, yi, zi, src_r;
+  uint64_t photons;
+  uint64_t seed;
+  int32_t fluorescence; /* 1: excitation + emission, 0: excitation only */
+} mcfl_request;
+
+typedef struct mcfl_request_result {
+  uint64_t* Rd_xy; /* det.nx*(det.ny+1) each; caller-owned */
+  uint64_t* Tt_xy;
+  uint64_t* FlR_raw; /* fluorescence requests */
+  uint64_t* FlT_raw;
+  double fl_unit;
+  uint64_t totals[MCFL_NTOT];
+  uint64_t fl_totals[MCFL_NTOT];
+  uint64_t epochs; /* epochs the request took part in */
+} mcfl_request_result;
+
+mcfl_status mcfl_server_create(mcfl_ctx* ctx, uint64_t mem_budget_bytes, uint64_t epoch_photons, mcfl_server** out);
+/* Makes a medium resident; the simulation's bulks/labels/detector are copied. */
+mcfl_status mcfl_server_add_medium(mcfl_server* s, const mcfl_sim* medium, int32_t* medium_id);
+/* Queues a request (admitted at the next step that has room). */
+mcfl_status mcfl_server_submit(mcfl_server* s, const mcfl_request* req, int64_t* request_id);
+/* Admits, runs one epoch, retires finished requests (their ids in done_ids). */
+mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32_t max_done, int32_t* n_done, double* kernel_ms);
+/* Copies a finished request's results and forgets it. */
+mcfl_status mcfl_server_collect(mcfl_server* s, int64_t request_id, mcfl_request_result* out);
+mcfl_status mcfl_server_stats(mcfl_server* s, int32_t* queued, int32_t* active, uint64_t* bytes_in_use);
+void mcfl_server_destroy(mcfl_server* s);
 
 /* ---- writers (byte layouts of the reference) ---- */
 

@@ -90,7 +90,8 @@ def test_fluorescence_bit_exact(ctx):
     assert np.array_equal(fl["raw_R"], r.Rd) and np.array_equal(fl["raw_T"], r.Tt)
     check_totals(fl["totals"], r.tot)
     assert fl["raw_R"].sum() > 0
-    np.testing.assert_allclose(fl["FlR"], r.Rd[:100] * unit / (0.1 * 0.1), rtol=1e-12)
+    dxdy = float(np.float32(0.1)) * float(np.float32(0.1))
+    np.testing.assert_allclose(fl["FlR"], r.Rd[:100] * unit / dxdy, rtol=1e-12)
 
 
 def test_glass_inclusion_bit_exact(ctx):
