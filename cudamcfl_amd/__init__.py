@@ -297,11 +297,15 @@ class Context:
     """A device context (mcfl_ctx): one GPU, its stream and resident buffers."""
 
     def __init__(self, device=0, primes=None):
+        import weakref
         self.h = C.c_void_p()
+        self._children = weakref.WeakSet()  # batches and servers: destroyed before the context
         check(lib().mcfl_ctx_create(device, (primes or "").encode(), C.byref(self.h)), "mcfl_ctx_create")
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                ch.close()
             lib().mcfl_ctx_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -451,6 +455,7 @@ class Batch:
         self.h = C.c_void_p()
         check(L.mcfl_batch_create(ctx.h, self._sarr, len(self.sims), self._jarr, len(self.jobs), C.byref(self.h)),
               "mcfl_batch_create")
+        ctx._children.add(self)
 
     def run(self, seeds=None):
         if seeds is not None:
@@ -483,6 +488,7 @@ class Server:
         _bind_engine(L)
         self.h = C.c_void_p()
         check(L.mcfl_server_create(ctx.h, int(mem_budget), int(epoch_photons), C.byref(self.h)), "mcfl_server_create")
+        ctx._children.add(self)
         self.media = []
 
     def add_medium(self, sim):

@@ -132,7 +132,8 @@ uint64_t mcfl_stream_key(uint64_t seed, uint64_t stream);
 
 typedef struct mcfl_ctx mcfl_ctx;
 
-/* Opens HIP device `device`, loads the MWC multiplier table (NULL/"" = bundled). */
+/* Opens HIP device `device`, loads the MWC multiplier table (NULL/"" = bundled).
+ * Destroy every batch and server of a context before the context. */
 mcfl_status mcfl_ctx_create(int device, const char* safeprimes_file, mcfl_ctx** out);
 void mcfl_ctx_destroy(mcfl_ctx* ctx);
 const char* mcfl_status_string(mcfl_status s);

@@ -140,7 +140,8 @@ def test_server_results_equal_alone(ctx):
 
 def test_server_admission_by_footprint(ctx):
     sim = medium(20, 20, 10, 0)
-    srv = M.Server(ctx, mem_budget=20000, epoch_photons=2000)  # room for one small request at a time
+    one = 2 * sim.npix * 8 + 16 * 8  # footprint of an excitation-only request: R and T images + totals
+    srv = M.Server(ctx, mem_budget=one + one // 2, epoch_photons=2000)  # room for one request at a time
     mid = srv.add_medium(sim)
     ids = [srv.submit(mid, 1500, s) for s in range(3)]
     srv.step()
