@@ -144,10 +144,9 @@ def test_server_admission_by_footprint(ctx):
     srv = M.Server(ctx, mem_budget=one + one // 2, epoch_photons=2000)  # room for one request at a time
     mid = srv.add_medium(sim)
     ids = [srv.submit(mid, 1500, s) for s in range(3)]
-    srv.step()
+    done, _ = srv.step()
     st = srv.stats()
     assert st["active"] <= 1 and st["queued"] >= 1
-    done = []
     for _ in range(50):
         d, _ = srv.step()
         done += d
