@@ -50,6 +50,156 @@ def cpu_baseline(target_s=12.0):
             "photon_steps_per_s": r.tot["steps"] / dt}
 
 
+def rse_stats(jobs):
+    """Per-pixel relative standard error of the reflectance images from the
+    sum and sum-of-squares tallies: RSE = sqrt(Q - S^2/N) / S, over pixels
+    above 1e-4 of each image's peak."""
+    vals = []
+    for j in jobs:
+        S = j.Rd.astype(np.float64)
+        Q = j.Rd2.astype(np.float64) * 2.0 ** 24
+        keep = S > 1e-4 * S.max()
+        r = np.sqrt(np.maximum(Q[keep] - S[keep] ** 2 / j.photons, 0.0)) / S[keep]
+        vals.append(r)
+    v = np.concatenate(vals)
+    return {"median": float(np.median(v)), "p90": float(np.percentile(v, 90)),
+            "frac_le_1pct": float(np.mean(v <= 0.01)), "pixels": int(v.size)}
+
+
+def bench_scan(ctx):
+    """64-source scan on the 128x128x64 medium: one graph-captured batched
+    launch vs 64 plain launches with the same budgets and seeds (bit-identical
+    images, hence identical per-pixel RSE)."""
+    import cudamcfl_amd as M
+    from cudamcfl_amd.media import cfg2_simulation, scan_sources, scan_budgets
+    sim = cfg2_simulation(photons=1000, fhd=False)
+    srcs, budgets = scan_sources(), scan_budgets()
+    total = int(sum(budgets))
+    jobs = [M.BatchJob(0, int(n), 700 + i, source=s, tallies=M.TALLY_SQ) for i, (s, n) in enumerate(zip(srcs, budgets))]
+    b = M.Batch(ctx, [sim], jobs)
+    b.run()
+    t0 = time.perf_counter()
+    kms = b.run()
+    wall_b = time.perf_counter() - t0
+    print(f"scan batched: {wall_b:.2f} s wall, {kms:.0f} ms device", file=sys.stderr, flush=True)
+    plain_wall, plain_ms, same = 0.0, 0.0, True
+    for i, (s, n) in enumerate(zip(srcs, budgets)):
+        sim.s.number_of_photons = int(n)
+        sim.s.xi, sim.s.yi, sim.s.zi = s
+        t0 = time.perf_counter()
+        r = ctx.run_excitation(sim, seed=700 + i, sq=True)
+        plain_wall += time.perf_counter() - t0
+        plain_ms += r.kernel_ms
+        same &= bool(np.array_equal(r.Rd, jobs[i].Rd) and np.array_equal(r.Rd2, jobs[i].Rd2))
+    print(f"scan plain: {plain_wall:.2f} s wall, {plain_ms:.0f} ms device, identical={same}", file=sys.stderr,
+          flush=True)
+    out = {"workload": "8x8 pencil scan, 8-voxel pitch, budgets log-uniform [1e6,5e7] (seed 7), 128x128x64 medium",
+           "photons": total, "batched_photons_per_s": total / wall_b, "batched_device_ms": kms,
+           "plain_photons_per_s": total / plain_wall, "plain_device_ms": plain_ms,
+           "speedup_wall": plain_wall / wall_b, "speedup_device": plain_ms / kms,
+           "images_identical": same, "rse": rse_stats(jobs)}
+    b.close()
+    return out
+
+
+def bench_hetero(ctx):
+    """32 heterogeneous simulations (5 grid sizes up to 256x256x128, own media,
+    pencil or disk sources, budgets log-uniform [1e5,1e8]) in one launch vs one
+    launch each."""
+    import cudamcfl_amd as M
+    from cudamcfl_amd.media import hetero_jobs
+    media, spec = hetero_jobs()
+    total = sum(j[4] for j in spec)
+    jobs = [M.BatchJob(m, n, seed, source=src, src_kind=kind, src_r=r) for (m, kind, r, src, n, seed) in spec]
+    b = M.Batch(ctx, media, jobs)
+    t0 = time.perf_counter()
+    kms = b.run()
+    wall_b = time.perf_counter() - t0
+    plain_wall, plain_ms, same = 0.0, 0.0, True
+    for (m, kind, r, src, n, seed), j in zip(spec, jobs):
+        s = media[m]
+        s.s.number_of_photons, s.s.src_kind, s.s.src_r = n, kind, r
+        s.s.xi, s.s.yi, s.s.zi = src
+        s.s.fhd_activated = 0
+        t0 = time.perf_counter()
+        res = ctx.run_excitation(s, seed=seed)
+        plain_wall += time.perf_counter() - t0
+        plain_ms += res.kernel_ms
+        same &= bool(np.array_equal(res.Rd, j.Rd))
+    b.close()
+    return {"workload": "32 media, grids 64^3..256x256x128, 1-6 inclusions, 4-8 materials, pencil/disk, "
+                        "budgets log-uniform [1e5,1e8] (seed 11)", "photons": int(total),
+            "batched_photons_per_s": total / wall_b, "batched_device_ms": kms,
+            "plain_photons_per_s": total / plain_wall, "plain_device_ms": plain_ms,
+            "speedup_wall": plain_wall / wall_b, "images_identical": same}
+
+
+def bench_serve(ctx, rate=3.0, duration=12.0, epoch=4_000_000):
+    """Poisson request stream: continuous batching vs a plain
+    one-launch-per-request FIFO loop on the same arrivals."""
+    import cudamcfl_amd as M
+    from cudamcfl_amd.media import request_stream
+    media, reqs = request_stream(rate=rate, duration=duration)
+    work = sum(n * (2 if fl else 1) for (_, _, _, n, _, fl) in reqs)
+
+    srv = M.Server(ctx, mem_budget=48 << 30, epoch_photons=epoch)
+    mids = [srv.add_medium(m) for m in media]
+    t0 = time.perf_counter()
+    arr, lat, i = {}, {}, 0
+    while i < len(reqs) or len(lat) < len(reqs):
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i][0] <= now:
+            ta, m, src, n, seed, fl = reqs[i]
+            rid = srv.submit(mids[m], n, seed, source=src, fluorescence=fl)
+            arr[rid] = (ta, mids[m])
+            i += 1
+        st = srv.stats()
+        if st["active"] == 0 and st["queued"] == 0:
+            if i < len(reqs):
+                time.sleep(max(0.0, reqs[i][0] - (time.perf_counter() - t0)))
+            continue
+        done, _ = srv.step()
+        tnow = time.perf_counter() - t0
+        for rid in done:
+            lat[rid] = tnow - arr[rid][0]
+            srv.collect(rid, arr[rid][1])
+    span_s = time.perf_counter() - t0
+    srv.close()
+    served = np.array(sorted(lat.values()))
+    print(f"served: {len(served)} requests, span {span_s:.1f} s", file=sys.stderr, flush=True)
+
+    # plain loop: requests in arrival order, one launch each (+ one emission launch)
+    t0 = time.perf_counter()
+    plat = []
+    for (ta, m, src, n, seed, fl) in reqs:
+        now = time.perf_counter() - t0
+        if now < ta:
+            time.sleep(ta - now)
+        s = media[m]
+        s.s.number_of_photons = n
+        s.s.xi, s.s.yi, s.s.zi = src
+        s.s.src_kind = M.SOURCE_PENCIL
+        s.s.fhd_activated = 1 if fl else 0
+        r = ctx.run_excitation(s, seed=seed)
+        if fl:
+            Fx = r.fhd.astype(np.float64) / (4294967295.0 * r.totals["photons"])
+            job = M.BatchJob(0, n, seed, src_kind=M.SOURCE_EMISSION, Fx=Fx)
+            bb = M.Batch(ctx, [s], [job])
+            bb.run()
+            bb.close()
+        plat.append(time.perf_counter() - t0 - ta)
+    span_p = time.perf_counter() - t0
+    plain = np.array(sorted(plat))
+    print(f"plain: span {span_p:.1f} s", file=sys.stderr, flush=True)
+    return {"workload": f"Poisson arrivals {rate}/s for {duration}s ({len(reqs)} requests), 8 media up to "
+                        f"128x128x64, scan sources, 50% with fluorescence, budgets log-uniform [1e5,1e8]",
+            "photons": int(work), "epoch_photons": epoch,
+            "served": {"p50_s": float(np.percentile(served, 50)), "p99_s": float(np.percentile(served, 99)),
+                       "photons_per_s": work / span_s, "span_s": span_s},
+            "plain": {"p50_s": float(np.percentile(plain, 50)), "p99_s": float(np.percentile(plain, 99)),
+                      "photons_per_s": work / span_p, "span_s": span_p}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +207,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scan", action="store_true", help="also run the 64-source batched scan")
+    ap.add_argument("--hetero", action="store_true", help="also run the 32-medium heterogeneous batch")
+    ap.add_argument("--serve", action="store_true", help="also run the served request stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -116,6 +269,12 @@ def main():
                      "note": f"{F_STEP} VALU issue slots per photon-step x photon-steps / kernel time"},
         "gpu": info,
     }
+    if args.scan:
+        out["batched_scan"] = bench_scan(ctx)
+    if args.hetero:
+        out["batched_hetero"] = bench_hetero(ctx)
+    if args.serve:
+        out["serving"] = bench_serve(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -9,7 +9,7 @@ Optical ranges are the benchmark's, converted from 1/mm to the reference's
 """
 import numpy as np
 
-from . import Simulation, SOURCE_PENCIL
+from . import Simulation, SOURCE_PENCIL, SOURCE_DISK
 
 # rows: n, mua, mus, flc, AbsCExi, AbsCEmi, eY, g (1/cm), .mci medium-line order
 
@@ -119,3 +119,77 @@ def write_mci(path, out_name, bulk_file, mats, esp, grid_size, photons, photons_
         lines.append(f"# mcfl-dims {dims[0]} {dims[1]} {dims[2]}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+# ---------------------------------------------------------------- workloads
+
+GRID_SIZES = [(64, 64, 64), (48, 96, 96), (64, 128, 128), (96, 192, 192), (128, 256, 256)]  # (nz, ny, nx)
+
+
+def scan_sources(n=8, spacing_vox=8, gs=20):
+    """8x8 pencil-beam positions spaced 8 voxels apart, centred (cm)."""
+    step = spacing_vox / gs
+    return [((i - (n - 1) / 2) * step, (j - (n - 1) / 2) * step, 0.0) for j in range(n) for i in range(n)]
+
+
+def log_uniform(rng, lo, hi, size):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size)).astype(np.int64)
+
+
+def scan_budgets(seed=7, n=64, lo=1e6, hi=5e7):
+    return log_uniform(np.random.default_rng(seed), lo, hi, n)
+
+
+def random_medium(rng, shape, gs=20):
+    """One heterogeneous medium: 1-6 inclusions (spheres or z-cylinders),
+    a 4-8 entry material table drawn from the benchmark ranges."""
+    n_mat = int(rng.integers(4, 9))
+    n_inc = int(rng.integers(1, 7))
+    nz, ny, nx = shape
+    n_sph = int(rng.integers(0, n_inc + 1))
+    lab = inclusions(shape, rng, n_spheres=n_sph, n_cyl=n_inc - n_sph)
+    # fold inclusion labels into the table size
+    lab = np.where(lab > 1, 2 + (lab - 2) % (n_mat - 1), lab).astype(np.uint8)
+    mats = random_materials(rng, n_mat)
+    return Simulation.voxel(lab, mats, esp=nz / gs, grid_size=gs, det=(1.0 / gs, 1.0 / gs, nx, ny, 0.0, 0.0),
+                            photons=1000, n_above=1.0, n_below=1.0, fhd=True)
+
+
+def hetero_jobs(seed=11, n=32, lo=1e5, hi=1e8, gs=20):
+    """32 independent simulations: grid sizes from GRID_SIZES, own media, a
+    pencil or a uniform disk (radius 2-10 voxels) at a random lateral
+    position, log-uniform budgets.  Returns (media, [(medium, kind, r, src, N, seed)])."""
+    rng = np.random.default_rng(seed)
+    media, jobs = [], []
+    budgets = log_uniform(rng, lo, hi, n)
+    for i in range(n):
+        shape = GRID_SIZES[int(rng.integers(0, len(GRID_SIZES)))]
+        media.append(random_medium(rng, shape, gs))
+        nz, ny, nx = shape
+        disk = bool(rng.integers(0, 2))
+        r = float(rng.uniform(2, 10)) / gs if disk else 0.0
+        src = (float(rng.uniform(-0.25, 0.25) * nx / gs), float(rng.uniform(-0.25, 0.25) * ny / gs), 0.0)
+        jobs.append((i, SOURCE_DISK if disk else SOURCE_PENCIL, r, src, int(budgets[i]), 1000 + i))
+    return media, jobs
+
+
+def request_stream(seed=5, rate=3.0, duration=12.0, n_media=8, lo=1e5, hi=1e8, gs=20):
+    """Poisson arrivals of forward simulations: media from the heterogeneous
+    generator (restricted to grids up to 128x128x64) and scan-style sources,
+    50% excitation-only / 50% excitation+fluorescence, log-uniform budgets.
+    Returns (media, [(t_arrival, medium, src, N, seed, fluorescence)])."""
+    rng = np.random.default_rng(seed)
+    media = [random_medium(rng, GRID_SIZES[int(rng.integers(0, 3))], gs) for _ in range(n_media)]
+    srcs = scan_sources(8, 8, gs)
+    reqs, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration:
+            break
+        m = int(rng.integers(0, n_media))
+        s = srcs[int(rng.integers(0, len(srcs)))]
+        nz, ny, nx = media[m].s.nz, media[m].s.ny, media[m].s.nx
+        s = (max(min(s[0], 0.45 * nx / gs), -0.45 * nx / gs), max(min(s[1], 0.45 * ny / gs), -0.45 * ny / gs), 0.0)
+        n = int(log_uniform(rng, lo, hi, 1)[0])
+        reqs.append((t, m, s, n, 5000 + len(reqs), bool(rng.integers(0, 2))))
+    return media, reqs
