@@ -18,6 +18,7 @@
 #define MCFL_MAX_JOBS 64         /* explicit jobs per launch (per-job totals live in LDS) */
 #define MCFL_NTOT 16             /* totals per job, see enum mcfl_total in mcfl.h */
 #define MCFL_MAX_STEPS_PER_PHOTON (1u << 28)
+#define MCFL_TALLY_COPIES 8      /* per-step volume tallies: one copy per XCD, summed after the launch */
 
 enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2 };
 enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2, MCFL_SRC_EMISSION = 3 };
@@ -87,6 +88,7 @@ struct JobDev {
   unsigned long long* Rd2;
   unsigned long long* Tt2;
   unsigned long long* tot; /* MCFL_NTOT counters */
+  uint64_t copy_stride; /* fhd/axyz: elements between the per-XCD copies (0: one copy) */
   DetDev det;
   MediumDev med; /* used when the launch mixes media */
   /* MCFL_SRC_EMISSION: Walker alias table over emitting voxels */

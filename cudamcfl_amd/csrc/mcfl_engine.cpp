@@ -97,6 +97,7 @@ void build_alias(const std::vector<double>& w, std::vector<uint32_t>& prob, std:
 struct Tallies {
   DevBuf Rd, Tt, fhd, axyz, Rd2, Tt2, time, tot, al_prob, al_alias, al_vox;
   uint32_t flags = 0;
+  uint32_t copies = 1; /* per-XCD copies of fhd / axyz */
   uint32_t al_n = 0;
   double fl_unit = 0.0;
   size_t bytes = 0;
@@ -105,8 +106,9 @@ struct Tallies {
   }
 };
 
-mcfl_status alloc_tallies(const Medium& m, uint32_t tallies, bool emission, Tallies* t) {
-  const size_t npix = m.npix * 8, nvox = m.nvox * 8;
+mcfl_status alloc_tallies(const Medium& m, uint32_t tallies, bool emission, uint32_t copies, Tallies* t) {
+  const size_t npix = m.npix * 8, nvox = m.nvox * 8 * copies;
+  t->copies = copies;
   MCFL_STCHK(ensure(t->Rd, npix));
   MCFL_STCHK(ensure(t->Tt, npix));
   MCFL_STCHK(ensure(t->tot, MCFL_NTOT * 8));
@@ -138,12 +140,20 @@ mcfl_status alloc_tallies(const Medium& m, uint32_t tallies, bool emission, Tall
 
 // Queue clearing of every tally buffer with the fill kernel (graph safe).
 mcfl_status clear_tallies(const Medium& m, Tallies* t, hipStream_t st) {
-  const uint64_t npix = m.npix, nvox = m.nvox;
+  const uint64_t npix = m.npix, nvox = m.nvox * t->copies;
   auto fill = [&](DevBuf& b, uint64_t n) { return b.p ? mcfl_fill_u64(b.as<unsigned long long>(), n, st) : 0; };
   if (fill(t->Rd, npix) || fill(t->Tt, npix) || fill(t->tot, MCFL_NTOT) || fill(t->fhd, nvox) ||
       fill(t->axyz, nvox) || fill(t->Rd2, npix) || fill(t->Tt2, npix) || fill(t->time, ntime_of(&m.sim)))
     return MCFL_ERR_DEVICE;
   return MCFL_OK;
+}
+
+// Sum the per-XCD copies of the volume tallies into copy 0.
+int reduce_tallies(const Medium& m, Tallies* t, hipStream_t st) {
+  int e = 0;
+  if (t->fhd.p) e |= mcfl_reduce_copies(t->fhd.as<unsigned long long>(), m.nvox, t->copies, st);
+  if (t->axyz.p) e |= mcfl_reduce_copies(t->axyz.as<unsigned long long>(), m.nvox, t->copies, st);
+  return e;
 }
 
 // Emission sources of a medium given its normalised fhd: weights
@@ -213,6 +223,7 @@ void fill_job(JobDev& J, const Medium& m, Tallies& t, int src_kind, float xi, fl
   J.Rd2 = t.Rd2.as<unsigned long long>();
   J.Tt2 = t.Tt2.as<unsigned long long>();
   J.tot = t.tot.as<unsigned long long>();
+  J.copy_stride = t.copies > 1 ? m.nvox : 0;
   fill_det(&m.sim, &J.det, m.d_tpos.as<const float>());
   fill_medium(&m.sim, m.d_labels.as<const uint8_t>(), &J.med);
   J.al_prob = t.al_prob.as<const uint32_t>();
@@ -331,7 +342,7 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
     b->tal.emplace_back(new Tallies());
     Tallies& t = *b->tal.back();
     const bool em = s.src_kind == MCFL_SOURCE_EMISSION;
-    MCFL_STCHK(alloc_tallies(m, s.tallies, em, &t));
+    MCFL_STCHK(alloc_tallies(m, s.tallies, em, c->copies, &t));
     if (em) MCFL_STCHK(build_emission(m, s.Fx, s.photons, &t));
     if (t.flags & MCFL_F_TEMP) b->temporal = 1;
     const uint64_t count = (em && t.al_n == 0) ? 0 : s.photons;
@@ -362,6 +373,8 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
   if (st == MCFL_OK && b->A.total > 0 &&
       mcfl_transport_launch(&b->A, b->temporal, b->perjob, b->nblocks, c->stream) != 0)
     st = MCFL_ERR_DEVICE;
+  for (int j = 0; j < n_jobs && st == MCFL_OK; ++j)
+    if (reduce_tallies(*b->media[jobs[j].medium], b->tal[j].get(), c->stream) != 0) st = MCFL_ERR_DEVICE;
   hipError_t e = hipStreamEndCapture(c->stream, &b->graph);
   if (st != MCFL_OK) return st;
   if (e != hipSuccess) return MCFL_ERR_DEVICE;
@@ -437,9 +450,9 @@ struct Request {
   double fl_unit = 0.0;
 };
 
-size_t footprint_of(const Medium& m, const mcfl_request& r) {
+size_t footprint_of(const Medium& m, const mcfl_request& r, uint32_t copies) {
   size_t b = 2 * m.npix * 8 + MCFL_NTOT * 8;
-  if (r.fluorescence) b += m.nvox * 8 + 2 * m.npix * 8 + MCFL_NTOT * 8 + 3 * 4 * m.nvox;
+  if (r.fluorescence) b += m.nvox * 8 * copies + 2 * m.npix * 8 + MCFL_NTOT * 8 + 3 * 4 * m.nvox;
   return b;
 }
 
@@ -488,7 +501,7 @@ extern "C" mcfl_status mcfl_server_submit(mcfl_server* s, const mcfl_request* r,
   q->id = s->next_id++;
   q->r = *r;
   q->m = s->media[r->medium].get();
-  q->footprint = footprint_of(*q->m, *r);
+  q->footprint = footprint_of(*q->m, *r, s->c->copies);
   if (q->footprint > s->budget) return MCFL_ERR_CAPACITY;
   *id = q->id;
   s->queue.push_back(std::move(q));
@@ -502,11 +515,11 @@ mcfl_status admit(mcfl_server* s) {
     Request& q = *s->queue.front();
     if (s->in_use + q.footprint > s->budget) break;  // admission by memory footprint, FIFO
     q.ex.reset(new Tallies());
-    MCFL_STCHK(alloc_tallies(*q.m, q.r.fluorescence ? (uint32_t)MCFL_TALLY_FHD : 0u, false, q.ex.get()));
+    MCFL_STCHK(alloc_tallies(*q.m, q.r.fluorescence ? (uint32_t)MCFL_TALLY_FHD : 0u, false, s->c->copies, q.ex.get()));
     MCFL_STCHK(clear_tallies(*q.m, q.ex.get(), s->c->stream));
     if (q.r.fluorescence) {
       q.em.reset(new Tallies());
-      MCFL_STCHK(alloc_tallies(*q.m, 0u, true, q.em.get()));
+      MCFL_STCHK(alloc_tallies(*q.m, 0u, true, s->c->copies, q.em.get()));
       MCFL_STCHK(clear_tallies(*q.m, q.em.get(), s->c->stream));
     }
     q.phase = 1;
@@ -525,6 +538,8 @@ mcfl_status advance(mcfl_server* s, Request& q) {
     q.Tt.resize(m.npix);
     MCFL_STCHK(download(m, *q.ex, q.Rd.data(), q.Tt.data(), nullptr, nullptr, nullptr, nullptr, nullptr, q.tot));
     if (q.r.fluorescence) {
+      if (reduce_tallies(m, q.ex.get(), s->c->stream) != 0) return MCFL_ERR_DEVICE;
+      MCFL_HIPCHK(hipStreamSynchronize(s->c->stream));
       std::vector<uint64_t> fhd(m.nvox);
       MCFL_HIPCHK(hipMemcpy(fhd.data(), q.ex->fhd.p, m.nvox * 8, hipMemcpyDeviceToHost));
       std::vector<double> Fx(m.nvox);

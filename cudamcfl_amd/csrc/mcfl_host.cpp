@@ -139,12 +139,14 @@ int grid_blocks(const mcfl_ctx* c, int work, int temporal, int perjob) {
   return c->n_cu * (bpc > 0 ? bpc : 1);
 }
 
-mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms) {
+mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms,
+                            const std::function<int()>& post) {
   const int nblocks = grid_blocks(c, A.work, temporal, perjob);
   A.n_waves = (uint32_t)nblocks * 4u;
   if (A.chunk_max == 0) A.chunk_max = 1024;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (mcfl_transport_launch(&A, temporal, perjob, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
+  if (post && post() != 0) return MCFL_ERR_DEVICE;
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));
   float t = 0.f;
@@ -284,6 +286,7 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   c->n_cu = prop.multiProcessorCount;
+  if (const char* e = getenv("MCFL_TALLY_COPIES")) c->copies = atoi(e) > 1 ? MCFL_TALLY_COPIES : 1;
   snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
   for (int w = 0; w < 3; ++w)
     for (int t = 0; t < 2; ++t)
@@ -371,15 +374,17 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
   J.Rd = static_cast<unsigned long long*>(c->Rd.p);
   J.Tt = static_cast<unsigned long long*>(c->Tt.p);
   J.tot = static_cast<unsigned long long*>(c->tot.p);
+  const uint32_t copies = c->copies;
+  J.copy_stride = copies > 1 ? nv : 0;
   if (fhd_on) {
-    STCHK(ensure(c->fhd, nv * 8));
-    HIPCHK(hipMemsetAsync(c->fhd.p, 0, nv * 8, c->stream));
+    STCHK(ensure(c->fhd, nv * 8 * copies));
+    HIPCHK(hipMemsetAsync(c->fhd.p, 0, nv * 8 * copies, c->stream));
     J.fhd = static_cast<unsigned long long*>(c->fhd.p);
     J.flags |= MCFL_F_FHD;
   }
   if (out->A_xyz) {
-    STCHK(ensure(c->axyz, nv * 8));
-    HIPCHK(hipMemsetAsync(c->axyz.p, 0, nv * 8, c->stream));
+    STCHK(ensure(c->axyz, nv * 8 * copies));
+    HIPCHK(hipMemsetAsync(c->axyz.p, 0, nv * 8 * copies, c->stream));
     J.axyz = static_cast<unsigned long long*>(c->axyz.p);
     J.flags |= MCFL_F_AXYZ;
   }
@@ -431,7 +436,13 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     A.work = MCFL_WORK_JOBS;
     A.total = s->number_of_photons;
   }
-  STCHK(launch_and_time(c, A, temporal, 0, &out->kernel_ms));
+  auto reduce = [&]() -> int {
+    int e = 0;
+    if (fhd_on) e |= mcfl_reduce_copies(c->fhd.as<unsigned long long>(), nv, copies, c->stream);
+    if (out->A_xyz) e |= mcfl_reduce_copies(c->axyz.as<unsigned long long>(), nv, copies, c->stream);
+    return e;
+  };
+  STCHK(launch_and_time(c, A, temporal, 0, &out->kernel_ms, reduce));
 
   HIPCHK(hipMemcpy(out->Rd_xy, c->Rd.p, npix * 8, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(out->Tt_xy, c->Tt.p, npix * 8, hipMemcpyDeviceToHost));

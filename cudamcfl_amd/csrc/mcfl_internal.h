@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include <hip/hip_runtime_api.h>
@@ -15,6 +16,7 @@
 extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int* blocks_per_cu);
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream);
 extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
+extern "C" int mcfl_reduce_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, hipStream_t stream);
 
 namespace mcfl_detail {
 
@@ -50,6 +52,7 @@ struct mcfl_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
   int bpc[3][2][2] = {};  // resident workgroups per CU [work][temporal][perjob]
+  uint32_t copies = MCFL_TALLY_COPIES;  // copies of the per-step volume tallies (1 or MCFL_TALLY_COPIES)
   char name[256] = {};
   uint32_t* d_atab = nullptr;
   std::vector<uint32_t> atab;
@@ -58,7 +61,9 @@ struct mcfl_ctx {
 
 namespace mcfl_detail {
 // Launch geometry, launch, HIP-event timing on the context stream.
-mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms);
+// `post` (may be empty) queues follow-up kernels (copy reductions) inside the timed region.
+mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms,
+                            const std::function<int()>& post = {});
 int grid_blocks(const mcfl_ctx* c, int work, int temporal, int perjob);
 }  // namespace mcfl_detail
 

@@ -18,7 +18,10 @@
  *  - Medium: 8-bit labels in HBM (L2/MALL resident), per-material constants
  *    in LDS.
  *  - Tallies: 64-bit integer atomics (order independent); per-job scalar
- *    totals are privatised in LDS and flushed once per workgroup.
+ *    totals are privatised in LDS and flushed once per workgroup.  The
+ *    per-step volume tallies (photon hitting density, absorbed weight) go to
+ *    one copy per XCD, chosen by the hardware XCC_ID, so their cache lines
+ *    never move between the XCDs' L2 caches; a reduce kernel sums the copies.
  *
  * The step follows CUDAMCFLtransport.cu:289-447 quirk for quirk (end-point
  * label lookup, z-normal Fresnel, absorption and spin at a surface point after
@@ -299,6 +302,8 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
   /* the medium: launch-uniform, or the current photon's job's (PERJOB) */
   MediumDev m = A.med;
   const uint32_t lane = __lane_id();
+  /* XCD of this workgroup (HW_REG_XCC_ID[3:0]); selects the volume-tally copy */
+  const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20) & (MCFL_TALLY_COPIES - 1);
 
   Ph p = {};
   uint64_t x = 0;
@@ -322,8 +327,8 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
   if (WORK == MCFL_WORK_VOXELS) {
     mbase = A.jobs[0].mat_base;
     flags = A.jobs[0].flags;
-    fhd = A.jobs[0].fhd;
-    axyz = A.jobs[0].axyz;
+    fhd = A.jobs[0].fhd ? A.jobs[0].fhd + xcc * A.jobs[0].copy_stride : nullptr;
+    axyz = A.jobs[0].axyz ? A.jobs[0].axyz + xcc * A.jobs[0].copy_stride : nullptr;
   }
 
   for (;;) {
@@ -365,8 +370,8 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
           x = mcfl_photon_state(J.key, k, a);
           mbase = J.mat_base;
           flags = J.flags;
-          fhd = J.fhd;
-          axyz = J.axyz;
+          fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
+          axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
           escale = J.exit_scale;
           if (PERJOB) m = J.med;
           if (J.src_kind == MCFL_SRC_EMISSION) {
@@ -403,8 +408,8 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
           lane_left = A.lane_q + ((uint64_t)lane_item < A.lane_rem ? 1ull : 0ull) - 1ull;
           mbase = J.mat_base;
           flags = J.flags;
-          fhd = J.fhd;
-          axyz = J.axyz;
+          fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
+          axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
           uint64_t xl = x; /* LaunchPhoton_Global does not store the advanced state */
           launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, xl, a);
         }
@@ -530,6 +535,15 @@ __global__ void mcfl_fill_u64_kernel(unsigned long long* p, uint64_t n) {
     p[i] = 0ull;
 }
 
+/* Sums copies 1..ncopies-1 of a per-XCD volume tally into copy 0. */
+__global__ void mcfl_reduce_copies_kernel(unsigned long long* p, uint64_t n, uint32_t ncopies) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long s = p[i];
+    for (uint32_t c = 1; c < ncopies; ++c) s += p[(uint64_t)c * n + i];
+    p[i] = s;
+  }
+}
+
 /* ------------------------------------------------------------------ host */
 
 namespace {
@@ -558,6 +572,14 @@ extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perj
   hipError_t e = hipLaunchKernel(kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0),
                                  dim3((unsigned)nblocks), dim3(MCFL_WG), args, 0, stream);
   return e == hipSuccess ? 0 : (int)e;
+}
+
+extern "C" int mcfl_reduce_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, hipStream_t stream) {
+  if (n == 0 || ncopies <= 1) return 0;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  mcfl_reduce_copies_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(p, n, ncopies);
+  return (int)hipGetLastError();
 }
 
 extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream) {

@@ -96,7 +96,8 @@ def write_mci_variant(tmp_path, **edits):
 def test_parser_clamps_photon_counts(tmp_path, monkeypatch):
     monkeypatch.chdir(REF_INPUTS)
     p = write_mci_variant(tmp_path, **{"500000000 ": "1000 ", "40000 ": "32768 "})
-    s = M.Simulation.from_mci(p).s
+    sim = M.Simulation.from_mci(p)
+    s = sim.s
     assert s.number_of_photons == 32767 and s.number_of_photons_per_voxel == 32767
 
 
