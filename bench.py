@@ -50,6 +50,20 @@ def cpu_baseline(target_s=12.0):
             "photon_steps_per_s": r.tot["steps"] / dt}
 
 
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size: the calibration point of
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def rse_stats(jobs):
     """Per-pixel relative standard error of the reflectance images from the
     sum and sum-of-squares tallies: RSE = sqrt(Q - S^2/N) / S, over pixels
@@ -207,6 +221,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="copy this many bytes device-to-device first")
     ap.add_argument("--scan", action="store_true", help="also run the 64-source batched scan")
     ap.add_argument("--hetero", action="store_true", help="also run the 32-medium heterogeneous batch")
     ap.add_argument("--serve", action="store_true", help="also run the served request stream")
@@ -221,6 +236,8 @@ def main():
     N = int(args.photons)
     sim = cfg2_simulation(photons=N)
     ctx = M.Context(0)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     info = ctx.info()
     for w in range(args.warmup):
         ctx.run_excitation(sim, seed=1000 + w)

@@ -119,9 +119,19 @@ struct LaunchArgs {
   unsigned long long* xs;
   const uint32_t* as;
   uint64_t lane_q, lane_rem;
+  /* LDS tile of one job's photon hitting density: voxels [t0, t0+tn) per axis */
+  uint32_t tile_n;   /* entries (0: no tile) */
+  uint32_t tile_job; /* the job whose fhd the tile privatises */
+  int32_t tx0, ty0, tz0;
+  uint32_t tnx, tny, tnz;
   /* launch geometry */
   uint32_t n_waves;
   uint32_t chunk_max;
 };
+
+/* Dynamic LDS of a launch: per-job totals, fhd tile, material table. */
+static inline uint32_t mcfl_lds_bytes(uint32_t nslots, uint32_t tile_n, uint32_t n_mats) {
+  return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * 32u;
+}
 
 #endif

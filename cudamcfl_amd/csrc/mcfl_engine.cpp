@@ -361,7 +361,15 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
     }
   b->perjob = n_used > 1;
   MCFL_STCHK(b->L.prepare(c, b->jd, b->mt, b->perjob, uni, &b->A));
-  b->nblocks = grid_blocks(c, MCFL_WORK_JOBS, b->temporal, b->perjob);
+  for (int j = 0; j < n_jobs; ++j)  // LDS fhd tile for the first job tallying fhd
+    if (b->jd[j].fhd != nullptr) {
+      const Medium& m = *b->media[jobs[j].medium];
+      MediumDev md;
+      fill_medium(&m.sim, m.d_labels.as<const uint8_t>(), &md);
+      choose_tile(c, b->A, md, jobs[j].xi, jobs[j].yi, (uint32_t)j, b->temporal, b->perjob);
+      break;
+    }
+  b->nblocks = grid_blocks(c, b->A, b->temporal, b->perjob);
   b->A.n_waves = (uint32_t)b->nblocks * 4u;
   MCFL_HIPCHK(hipStreamSynchronize(c->stream));
 

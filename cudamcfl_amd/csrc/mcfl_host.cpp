@@ -8,6 +8,7 @@
 // one persistent kernel launch with no host polling.
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -134,14 +135,56 @@ int src_kind_of(const mcfl_sim* s) {
   return s->src_kind == MCFL_SOURCE_DISK ? MCFL_SRC_DISK : MCFL_SRC_PENCIL;
 }
 
-int grid_blocks(const mcfl_ctx* c, int work, int temporal, int perjob) {
-  const int bpc = c->bpc[work][temporal ? 1 : 0][perjob ? 1 : 0];
-  return c->n_cu * (bpc > 0 ? bpc : 1);
+int grid_blocks(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob) {
+  const uint32_t nslots = A.work == MCFL_WORK_JOBS ? A.njobs : 1u;
+  int bpc = 0;
+  if (mcfl_transport_occupancy(A.work, temporal, perjob, mcfl_lds_bytes(nslots, A.tile_n, A.n_mats), &bpc) != 0 ||
+      bpc <= 0)
+    bpc = 1;
+  return c->n_cu * bpc;
+}
+
+void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi, float yi, uint32_t job, int temporal,
+                 int perjob) {
+  A.tile_n = 0;
+  const uint32_t nslots = A.work == MCFL_WORK_JOBS ? A.njobs : 1u;
+  const int bpc = c->bpc[A.work][temporal ? 1 : 0][perjob ? 1 : 0];
+  if (bpc <= 0) return;
+  const size_t per_wg = c->lds_per_cu / (size_t)bpc;
+  const size_t other = mcfl_lds_bytes(nslots, 0, A.n_mats);
+  if (per_wg <= other + 64 * 8) return;
+  uint32_t cap = (uint32_t)std::min<size_t>((per_wg - other) / 8, 4096);
+  const uint32_t tnx = (uint32_t)std::min(m.nx, 16), tny = (uint32_t)std::min(m.ny, 16);
+  const uint32_t tnz = std::min<uint32_t>((uint32_t)m.nz, cap / (tnx * tny));
+  if (tnz == 0) return;
+  auto clampi = [](long v, long lo, long hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  const long ix = clampi((long)std::floor((xi + m.halfx) * m.gs), 0, m.nx - 1);
+  const long iy = clampi((long)std::floor((yi + m.halfy) * m.gs), 0, m.ny - 1);
+  A.tnx = tnx;
+  A.tny = tny;
+  A.tnz = tnz;
+  A.tx0 = (int32_t)clampi(ix - (long)tnx / 2, 0, m.nx - (long)tnx);
+  A.ty0 = (int32_t)clampi(iy - (long)tny / 2, 0, m.ny - (long)tny);
+  A.tz0 = 0;
+  A.tile_job = job;
+  A.tile_n = tnx * tny * tnz;
+  // keep the register-limited occupancy
+  int got = 0;
+  while (A.tile_n > 0 &&
+         (mcfl_transport_occupancy(A.work, temporal, perjob, mcfl_lds_bytes(nslots, A.tile_n, A.n_mats), &got) != 0 ||
+          got < bpc)) {
+    if (A.tnz <= 1) {
+      A.tile_n = 0;
+      break;
+    }
+    A.tnz -= 1;
+    A.tile_n = A.tnx * A.tny * A.tnz;
+  }
 }
 
 mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms,
                             const std::function<int()>& post) {
-  const int nblocks = grid_blocks(c, A.work, temporal, perjob);
+  const int nblocks = grid_blocks(c, A, temporal, perjob);
   A.n_waves = (uint32_t)nblocks * 4u;
   if (A.chunk_max == 0) A.chunk_max = 1024;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -291,7 +334,8 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   for (int w = 0; w < 3; ++w)
     for (int t = 0; t < 2; ++t)
       for (int pj = 0; pj < 2; ++pj)
-        if (mcfl_transport_occupancy(w, t, pj, &c->bpc[w][t][pj]) != 0) return fail(MCFL_ERR_DEVICE);
+        if (mcfl_transport_occupancy(w, t, pj, 0, &c->bpc[w][t][pj]) != 0) return fail(MCFL_ERR_DEVICE);
+  if (prop.maxSharedMemoryPerMultiProcessor > 0) c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   if (!primes || !primes[0]) primes = mcfl_default_primes_path();
@@ -436,6 +480,7 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     A.work = MCFL_WORK_JOBS;
     A.total = s->number_of_photons;
   }
+  if (fhd_on) choose_tile(c, A, A.med, s->xi, s->yi, 0, temporal, 0);
   auto reduce = [&]() -> int {
     int e = 0;
     if (fhd_on) e |= mcfl_reduce_copies(c->fhd.as<unsigned long long>(), nv, copies, c->stream);

@@ -13,7 +13,7 @@
 #include "../../include/mcfl.h"
 #include "mcfl_device.h"
 
-extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int* blocks_per_cu);
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, uint32_t lds_bytes, int* blocks_per_cu);
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream);
 extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
 extern "C" int mcfl_reduce_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, hipStream_t stream);
@@ -51,7 +51,8 @@ struct mcfl_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
-  int bpc[3][2][2] = {};  // resident workgroups per CU [work][temporal][perjob]
+  int bpc[3][2][2] = {};  // resident workgroups per CU [work][temporal][perjob], register-limited
+  size_t lds_per_cu = 160 * 1024;
   uint32_t copies = MCFL_TALLY_COPIES;  // copies of the per-step volume tallies (1 or MCFL_TALLY_COPIES)
   char name[256] = {};
   uint32_t* d_atab = nullptr;
@@ -64,7 +65,11 @@ namespace mcfl_detail {
 // `post` (may be empty) queues follow-up kernels (copy reductions) inside the timed region.
 mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms,
                             const std::function<int()>& post = {});
-int grid_blocks(const mcfl_ctx* c, int work, int temporal, int perjob);
+int grid_blocks(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob);
+// Chooses the fhd LDS tile of job `job` (box around its source voxel) within
+// the LDS left free at the kernel's register-limited occupancy.
+void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi, float yi, uint32_t job, int temporal,
+                 int perjob);
 }  // namespace mcfl_detail
 
 #define MCFL_HIPCHK(expr)                           \

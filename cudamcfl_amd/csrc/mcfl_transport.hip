@@ -290,13 +290,16 @@ __device__ __forceinline__ uint32_t find_job(const LaunchArgs& A, uint64_t g, ui
 
 template <int WORK, bool TEMPORAL, bool PERJOB>
 __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
-  __shared__ MatDev smat[MCFL_MAX_MATERIALS];
-  __shared__ unsigned long long stot[MCFL_MAX_JOBS * MCFL_NTOT];
+  /* dynamic LDS: [per-job totals | fhd tile | materials] */
+  extern __shared__ __align__(16) unsigned long long lds[];
 
   const uint32_t tid = threadIdx.x;
   const uint32_t nslots = (WORK == MCFL_WORK_JOBS) ? A.njobs : 1u;
+  unsigned long long* stot = lds;
+  unsigned long long* stile = lds + nslots * MCFL_NTOT;
+  MatDev* smat = reinterpret_cast<MatDev*>(stile + A.tile_n);
   for (uint32_t i = tid; i < A.n_mats; i += MCFL_WG) smat[i] = A.mats[i];
-  for (uint32_t i = tid; i < nslots * MCFL_NTOT; i += MCFL_WG) stot[i] = 0ull;
+  for (uint32_t i = tid; i < nslots * MCFL_NTOT + A.tile_n; i += MCFL_WG) lds[i] = 0ull;
   __syncthreads();
 
   /* the medium: launch-uniform, or the current photon's job's (PERJOB) */
@@ -315,6 +318,12 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
   unsigned long long* axyz = nullptr;
   double escale = 0.0;
   uint64_t accA = 0;
+  /* Run-length aggregation of the volume tallies: a photon takes many steps
+     inside one voxel (voxel size / mean free path ~ 6 and more), so each lane
+     sums its contributions while it stays in a voxel and issues one atomic
+     when it leaves the voxel or dies.  Integer sums are unchanged. */
+  uint32_t fv = 0xFFFFFFFFu, av = 0xFFFFFFFFu, ft = 0xFFFFFFFFu;
+  unsigned long long facc = 0ull, aacc = 0ull;
   uint32_t iters = 0;
   bool alive = false, retired = false;
   uint32_t lane_item = 0;
@@ -459,7 +468,26 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
     }
     if (nb != p.lab) reflect(mats, p, nb, x, a);
 
-    if ((flags & MCFL_F_FHD) && in) atomic_add_u64(fhd + vi, p.w);
+    if ((flags & MCFL_F_FHD) && in) {
+      if (vi != fv) {
+        if (facc != 0ull) {
+          if (ft != 0xFFFFFFFFu)
+            lds_add_u64(stile + ft, facc);
+          else
+            atomic_add_u64(fhd + fv, facc);
+        }
+        fv = vi;
+        facc = 0ull;
+        ft = 0xFFFFFFFFu;
+        if (A.tile_n != 0u && job == A.tile_job) { /* inside the LDS tile? */
+          const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
+          const uint32_t iz = vi / nxy, r2 = vi - iz * nxy, iy = r2 / (uint32_t)m.nx, ix = r2 - iy * (uint32_t)m.nx;
+          const uint32_t lx = ix - (uint32_t)A.tx0, ly = iy - (uint32_t)A.ty0, lz = iz - (uint32_t)A.tz0;
+          if (lx < A.tnx && ly < A.tny && lz < A.tnz) ft = lx + A.tnx * (ly + A.tny * lz);
+        }
+      }
+      facc += p.w;
+    }
 
     if (p.lab == 0u || p.lab == m.last) {
       const JobDev& J = A.jobs[job];
@@ -473,7 +501,14 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
       if (w > 0u) {
         p.w -= w;
         accA += w;
-        if ((flags & MCFL_F_AXYZ) && in) atomic_add_u64(axyz + vi, w);
+        if ((flags & MCFL_F_AXYZ) && in) {
+          if (vi != av) {
+            if (aacc != 0ull) atomic_add_u64(axyz + av, aacc);
+            av = vi;
+            aacc = 0ull;
+          }
+          aacc += w;
+        }
       } else {
         lds_add_u64(&st[MCFL_DT_ZERO_KILL], p.w);
         p.w = 0u;
@@ -502,6 +537,16 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
       lives = false;
     }
     if (!lives) {
+      if (facc != 0ull) {
+        if (ft != 0xFFFFFFFFu)
+          lds_add_u64(stile + ft, facc);
+        else
+          atomic_add_u64(fhd + fv, facc);
+      }
+      ft = 0xFFFFFFFFu;
+      if (aacc != 0ull) atomic_add_u64(axyz + av, aacc);
+      fv = av = 0xFFFFFFFFu;
+      facc = aacc = 0ull;
       lds_add_u64(&st[MCFL_DT_A], accA);
       lds_add_u64(&st[MCFL_DT_STEPS], iters);
       alive = false;
@@ -526,6 +571,19 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
   for (uint32_t i = tid; i < nslots * MCFL_NTOT; i += MCFL_WG) {
     const unsigned long long v = stot[i];
     if (v != 0ull) atomic_add_u64(A.jobs[i / MCFL_NTOT].tot + (i % MCFL_NTOT), v);
+  }
+  if (A.tile_n != 0u) {
+    const JobDev& J = A.jobs[A.tile_job];
+    unsigned long long* dst = J.fhd + xcc * J.copy_stride;
+    for (uint32_t i = tid; i < A.tile_n; i += MCFL_WG) {
+      const unsigned long long v = stile[i];
+      if (v == 0ull) continue;
+      const uint32_t lx = i % A.tnx, r2 = i / A.tnx, ly = r2 % A.tny, lz = r2 / A.tny;
+      const MediumDev& tm = PERJOB ? J.med : A.med; /* the tile job's medium, not the lane's last one */
+      const uint32_t v_idx =
+          (uint32_t)A.tx0 + lx + (uint32_t)tm.nx * ((uint32_t)A.ty0 + ly + (uint32_t)tm.ny * ((uint32_t)A.tz0 + lz));
+      atomic_add_u64(dst + v_idx, v);
+    }
   }
 }
 
@@ -561,16 +619,19 @@ const void* kernel_of(int work, int temporal, int perjob) {
 }
 }  // namespace
 
-extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int* blocks_per_cu) {
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel_of(work, temporal, perjob), MCFL_WG, 0);
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, uint32_t lds_bytes, int* blocks_per_cu) {
+  hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel_of(work, temporal, perjob), MCFL_WG, lds_bytes);
   return e == hipSuccess ? 0 : (int)e;
 }
 
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream) {
   if (A->work < 0 || A->work > 2) return -1;
   void* args[] = {const_cast<LaunchArgs*>(A)};
+  const uint32_t nslots = A->work == MCFL_WORK_JOBS ? A->njobs : 1u;
   hipError_t e = hipLaunchKernel(kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0),
-                                 dim3((unsigned)nblocks), dim3(MCFL_WG), args, 0, stream);
+                                 dim3((unsigned)nblocks), dim3(MCFL_WG), args,
+                                 mcfl_lds_bytes(nslots, A->tile_n, A->n_mats), stream);
   return e == hipSuccess ? 0 : (int)e;
 }
 
