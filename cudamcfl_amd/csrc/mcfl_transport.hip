@@ -121,19 +121,21 @@ __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint3
   mcfl_sincos_turn(mcfl_rand_co(&x, a), &sps, &cps);
   const float cosp = sps, sinp = cps; /* the reference's __sincosf argument order */
   const float dx = p.dx, dy = p.dy, dz = p.dz;
-  float tt = mcfl_sqrt(mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)));
+  const float q = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)); /* sin^2 of the polar angle; 0 or >= 6e-8 */
   float nx, ny, nz;
-  if (tt == 0.0f) {
+  if (q == 0.0f) { /* the reference's temp == 0 test on sqrt(q) */
     nx = sint * cosp;
     ny = sint * sinp;
     nz = mcfl_copysign(cost, dz * cost);
   } else {
-    float stt = sint * (1.0f / tt);
+    const float itt = mcfl_rsqrt(q); /* 1/temp */
+    const float tt = q * itt;        /* temp */
+    const float stt = sint * itt;
     nx = mcfl_fma(stt, dx * dz * cosp - dy * sinp, dx * cost);
     ny = mcfl_fma(stt, dy * dz * cosp + dx * sinp, dy * cost);
     nz = mcfl_fma(-sint * cosp, tt, dz * cost);
   }
-  float r = 1.0f / mcfl_sqrt(nx * nx + ny * ny + nz * nz);
+  const float r = mcfl_rsqrt(nx * nx + ny * ny + nz * nz);
   p.dx = nx * r;
   p.dy = ny * r;
   p.dz = nz * r;
@@ -466,7 +468,11 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
       in = in_grid(m, p.x, p.y, p.z);
       vi = in ? vox_of(m, p.x, p.y, p.z) : 0u;
     }
-    if (nb != p.lab) reflect(mats, p, nb, x, a);
+    MatDev mm = mc; /* material after a possible interface crossing */
+    if (nb != p.lab) {
+      reflect(mats, p, nb, x, a);
+      mm = mats[p.lab];
+    }
 
     if ((flags & MCFL_F_FHD) && in) {
       if (vi != fv) {
@@ -496,7 +502,6 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
       s = 0.0f;
     }
     if (s > 0.0f) {
-      const MatDev mm = mats[p.lab];
       const uint32_t w = mcfl_f2u_ru(mm.mua_mutr * (float)p.w);
       if (w > 0u) {
         p.w -= w;

@@ -176,20 +176,23 @@ static void orc_spin(const mat_t* m, orc_photon* p, uint64_t* x, uint32_t a) {
   float cosp = s_psi; /* :603 assigns the sine to cosp */
   float sinp = c_psi;
   float dx = p->dx, dy = p->dy, dz = p->dz;
-  float tt = mcfl_sqrt(mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)));
+  /* temp = sqrt(1 - dz^2) (:605); 1/temp and temp from one reciprocal root */
+  float s2 = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f));
   float ndx, ndy, ndz;
-  if (tt == 0.0f) {
+  if (s2 == 0.0f) { /* :607-612 normal incidence */
     ndx = sint * cosp;
     ndy = sint * sinp;
     ndz = mcfl_copysign(cost, dz * cost);
-  } else {
-    float inv = 1.0f / tt;
+  } else { /* :613-618 */
+    float inv = mcfl_rsqrt(s2);
+    float tt = s2 * inv;
     float st = sint * inv;
     ndx = mcfl_fma(st, dx * dz * cosp - dy * sinp, dx * cost);
     ndy = mcfl_fma(st, dy * dz * cosp + dx * sinp, dy * cost);
     ndz = mcfl_fma(-sint * cosp, tt, dz * cost);
   }
-  float r = 1.0f / mcfl_sqrt(ndx * ndx + ndy * ndy + ndz * ndz);
+  /* :621-624 renormalisation with a reciprocal square root */
+  float r = mcfl_rsqrt(ndx * ndx + ndy * ndy + ndz * ndz);
   p->dx = ndx * r;
   p->dy = ndy * r;
   p->dz = ndz * r;
@@ -719,6 +722,7 @@ void orc_math_eval(int fn, const float* in, float* out, size_t n) {
       case 1: mcfl_sincos_turn(in[i], &s, &c); out[i] = s; break;
       case 2: mcfl_sincos_turn(in[i], &s, &c); out[i] = c; break;
       case 3: { uint32_t u; memcpy(&u, &in[i], 4); out[i] = mcfl_u2f_rz(u); } break;
+      case 4: out[i] = mcfl_rsqrt(in[i]); break;
       default: out[i] = 0.0f;
     }
   }

@@ -139,7 +139,7 @@ void orc_spin_test(float g, const float* dir_in, uint64_t* x, uint32_t a, float*
 void orc_backstep_test(const float* p, const float* d, const float* c, float r2, float* t1, float* t2);
 float orc_fresnel_test(float n1, float n2, float cos_i);
 
-/* fn: 0 log, 1 sin(2 pi t), 2 cos(2 pi t), 3 uint32 (bits of in) -> float toward zero */
+/* fn: 0 log, 1 sin(2 pi t), 2 cos(2 pi t), 3 uint32 (bits of in) -> float toward zero, 4 1/sqrt */
 void orc_math_eval(int fn, const float* in, float* out, size_t n);
 /* n draws of the [0,1) generator from state x: states after each step and the draws */
 void orc_mwc_draws(uint64_t x, uint32_t a, size_t n, uint64_t* states, float* co);

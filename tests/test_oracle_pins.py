@@ -53,6 +53,16 @@ def test_sincos_turn_error_bound():
     assert np.array_equal(np.abs(O.math_eval(2, q)), [1, 0, 1, 0])
 
 
+def test_rsqrt_error_bound():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(6e-8, 1.0, 300000), rng.uniform(0.9, 1.1, 100000),
+                        np.geomspace(1e-30, 1e30, 20000)]).astype(np.float32)
+    got = O.math_eval(4, x).astype(np.float64)
+    ref = 1.0 / np.sqrt(x.astype(np.float64))
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(got - ref) / ulp) <= 2.0
+
+
 def test_u32_to_float_toward_zero():
     rng = np.random.default_rng(2)
     u = np.concatenate([rng.integers(0, 2 ** 32, 100000, dtype=np.uint64),
