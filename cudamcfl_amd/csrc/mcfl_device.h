@@ -50,6 +50,7 @@ struct MatDev {
 struct MediumDev {
   const uint8_t* labels; /* nx*ny*nz, x fastest */
   int32_t nx, ny, nz;
+  uint32_t nxy;  /* nx*ny */
   uint32_t last; /* n_bulks + 1 */
   float esp, gs, halfx, halfy;
 };
@@ -127,6 +128,7 @@ struct LaunchArgs {
   /* launch geometry */
   uint32_t n_waves;
   uint32_t chunk_max;
+  uint32_t refill_min; /* a wave refills when this many of its lanes are idle (1..64) */
 };
 
 /* Dynamic LDS of a launch: per-job totals, fhd tile, material table. */

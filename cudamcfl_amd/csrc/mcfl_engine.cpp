@@ -296,6 +296,7 @@ struct Launcher {
     A->jobs = d_jobs.as<const JobDev>();
     A->njobs = (uint32_t)jd.size();
     A->chunk_max = 1024;
+    A->refill_min = c->refill_min;
     return MCFL_OK;
   }
 };

@@ -82,6 +82,7 @@ mcfl_status validate(const mcfl_sim* s) {
   if (s->do_temp_sim && (s->det.temp_bins == 0 || s->det.max_temp == 0)) return MCFL_ERR_ARG;
   const size_t nv = nvox_of(s);
   if (nv > 0xFFFFFFFFull) return MCFL_ERR_CAPACITY;
+  if (s->nx >= (1 << 24) || s->ny >= (1 << 24)) return MCFL_ERR_CAPACITY;  // 24-bit index products in the kernel
   const uint8_t last = (uint8_t)(s->n_bulks + 1);
   for (size_t i = 0; i < nv; ++i)
     if (s->bulk_info[i] > last) return MCFL_ERR_LABEL;
@@ -93,6 +94,7 @@ void fill_medium(const mcfl_sim* s, const uint8_t* d_labels, MediumDev* md) {
   md->nx = s->nx;
   md->ny = s->ny;
   md->nz = s->nz;
+  md->nxy = (uint32_t)s->nx * (uint32_t)s->ny;
   md->last = s->n_bulks + 1;
   md->esp = s->esp;
   md->gs = (float)s->grid_size;
@@ -187,6 +189,7 @@ mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob
   const int nblocks = grid_blocks(c, A, temporal, perjob);
   A.n_waves = (uint32_t)nblocks * 4u;
   if (A.chunk_max == 0) A.chunk_max = 1024;
+  A.refill_min = c->refill_min;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (mcfl_transport_launch(&A, temporal, perjob, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
   if (post && post() != 0) return MCFL_ERR_DEVICE;
@@ -330,6 +333,7 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   c->n_cu = prop.multiProcessorCount;
   if (const char* e = getenv("MCFL_TALLY_COPIES")) c->copies = atoi(e) > 1 ? MCFL_TALLY_COPIES : 1;
+  if (const char* e = getenv("MCFL_REFILL_MIN")) c->refill_min = (uint32_t)std::min(64, std::max(1, atoi(e)));
   snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
   for (int w = 0; w < 3; ++w)
     for (int t = 0; t < 2; ++t)

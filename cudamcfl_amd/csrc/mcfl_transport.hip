@@ -61,8 +61,29 @@ __device__ __forceinline__ uint32_t vox_of(const MediumDev& m, float x, float y,
   return ix + (uint32_t)m.nx * (iy + (uint32_t)m.ny * iz);
 }
 
+/* Cell of an in-grid point (|x| < halfx, |y| < halfy, z < esp), the same
+   integers as vox_of: x + halfx and y + halfy are >= 0, so the truncating
+   conversion needs no saturation; z can be a rounding below 0 on the top
+   surface, hence the max. */
+struct Cell {
+  uint32_t ix, iy, iz;
+};
+__device__ __forceinline__ Cell cell_in(const MediumDev& m, float x, float y, float z) {
+  Cell c;
+  c.ix = min((uint32_t)((x + m.halfx) * m.gs), (uint32_t)m.nx - 1u);
+  c.iy = min((uint32_t)((y + m.halfy) * m.gs), (uint32_t)m.ny - 1u);
+  c.iz = min((uint32_t)mcfl_fmax(z * m.gs, 0.0f), (uint32_t)m.nz - 1u);
+  return c;
+}
+/* nx, ny < 2^24 and nx*ny*nz < 2^32 (host validation) keep the products exact */
+__device__ __forceinline__ uint32_t cell_index(const MediumDev& m, Cell c) {
+  return c.ix + __umul24(c.iy, (uint32_t)m.nx) + c.iz * m.nxy;
+}
+
+typedef __attribute__((address_space(1))) unsigned long long gu64;
+
 __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add((gu64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void lds_add_u64(unsigned long long* p, unsigned long long v) {
@@ -346,6 +367,10 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
     /* ---- refill: ballot the lanes without a photon, hand out indices ---- */
     bool need = !alive && !retired;
     uint64_t needmask = __ballot(need);
+    /* Refilling costs a wave the same whether one lane or many take a photon,
+       so idle lanes wait until refill_min of them are idle (or none is busy). */
+    if (needmask != 0ull && (uint32_t)__popcll(needmask) < A.refill_min && __ballot(alive) != 0ull && !pool_dead)
+      needmask = 0ull;
     while (needmask != 0ull) {
       if (pool_next == pool_end) {
         if (pool_dead) break;
@@ -431,7 +456,7 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
       pool_next += take;
       needmask = __ballot(need);
     }
-    if (need) retired = true;
+    if (need && pool_dead) retired = true;
     if (__ballot(alive) == 0ull) break;
     if (!alive) continue;
 
@@ -443,6 +468,7 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
     uint32_t nb;
     bool in;
     uint32_t vi;
+    Cell cl;
     ++iters;
     if (mc.mutr != kFltMax) {
       s = -mcfl_log(mcfl_rand_oc(&x, a)) * mc.mutr;
@@ -460,13 +486,15 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
       p.z = mcfl_fma(p.dz, s, p.z);
       if (TEMPORAL) p.tof += mcfl_d2u64_rz((double)(s * mc.n) / kCcmfs);
       in = in_grid(m, p.x, p.y, p.z);
-      vi = in ? vox_of(m, p.x, p.y, p.z) : 0u;
+      cl = cell_in(m, p.x, p.y, p.z);
+      vi = in ? cell_index(m, cl) : 0u;
       if (nb != 0u && nb != m.last) nb = in ? (uint32_t)m.labels[vi] : 1u;
     } else {
       s = 100.0f;
       nb = glass_march<TEMPORAL>(m, p, p.lab, s);
       in = in_grid(m, p.x, p.y, p.z);
-      vi = in ? vox_of(m, p.x, p.y, p.z) : 0u;
+      cl = cell_in(m, p.x, p.y, p.z);
+      vi = in ? cell_index(m, cl) : 0u;
     }
     MatDev mm = mc; /* material after a possible interface crossing */
     if (nb != p.lab) {
@@ -486,10 +514,8 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
         facc = 0ull;
         ft = 0xFFFFFFFFu;
         if (A.tile_n != 0u && job == A.tile_job) { /* inside the LDS tile? */
-          const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
-          const uint32_t iz = vi / nxy, r2 = vi - iz * nxy, iy = r2 / (uint32_t)m.nx, ix = r2 - iy * (uint32_t)m.nx;
-          const uint32_t lx = ix - (uint32_t)A.tx0, ly = iy - (uint32_t)A.ty0, lz = iz - (uint32_t)A.tz0;
-          if (lx < A.tnx && ly < A.tny && lz < A.tnz) ft = lx + A.tnx * (ly + A.tny * lz);
+          const uint32_t lx = cl.ix - (uint32_t)A.tx0, ly = cl.iy - (uint32_t)A.ty0, lz = cl.iz - (uint32_t)A.tz0;
+          if (lx < A.tnx && ly < A.tny && lz < A.tnz) ft = lx + __umul24(A.tnx, ly + __umul24(A.tny, lz));
         }
       }
       facc += p.w;
