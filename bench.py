@@ -38,10 +38,15 @@ def cpu_baseline(target_s=12.0):
     o = O.FromProduct(sim)
     a_tab = M.multipliers(32768)
     key = M.stream_key(99, 0)
-    t0 = time.perf_counter()
-    r = O.run_photons(o.o, a_tab, key, 0, 4000, nthreads=threads)
-    dt = time.perf_counter() - t0
-    n = int(max(4000, min(5_000_000, 4000 * target_s / max(dt, 1e-3))))
+    n = 4000
+    for _ in range(3):  # calibrate the sample size on growing runs (thread start-up dominates tiny ones)
+        t0 = time.perf_counter()
+        r = O.run_photons(o.o, a_tab, key, 0, n, nthreads=threads)
+        dt = time.perf_counter() - t0
+        if dt > 0.5:
+            break
+        n *= 8
+    n = int(max(4000, min(40_000_000, n * target_s / max(dt, 1e-3))))
     t0 = time.perf_counter()
     r = O.run_photons(o.o, a_tab, key, 0, n, nthreads=threads)
     dt = time.perf_counter() - t0
@@ -107,11 +112,28 @@ def bench_scan(ctx):
         same &= bool(np.array_equal(r.Rd, jobs[i].Rd) and np.array_equal(r.Rd2, jobs[i].Rd2))
     print(f"scan plain: {plain_wall:.2f} s wall, {plain_ms:.0f} ms device, identical={same}", file=sys.stderr,
           flush=True)
+    # the reference's launch geometry: 32768 MWC lanes per simulation (DoOneSimulation)
+    lanes_wall, lanes_ms, lane_res = 0.0, 0.0, []
+    for i, (s, n) in enumerate(zip(srcs, budgets)):
+        sim.s.number_of_photons = int(n)
+        sim.s.xi, sim.s.yi, sim.s.zi = s
+        x, a = M.init_rng(M.NUM_THREADS, seed=900 + i)
+        t0 = time.perf_counter()
+        r = ctx.run_excitation(sim, lanes=(x, a), sq=True)
+        lanes_wall += time.perf_counter() - t0
+        lanes_ms += r.kernel_ms
+        r.photons = int(n)
+        lane_res.append(r)
+    print(f"scan plain, reference lanes: {lanes_wall:.2f} s wall, {lanes_ms:.0f} ms device", file=sys.stderr,
+          flush=True)
     out = {"workload": "8x8 pencil scan, 8-voxel pitch, budgets log-uniform [1e6,5e7] (seed 7), 128x128x64 medium",
            "photons": total, "batched_photons_per_s": total / wall_b, "batched_device_ms": kms,
            "plain_photons_per_s": total / plain_wall, "plain_device_ms": plain_ms,
            "speedup_wall": plain_wall / wall_b, "speedup_device": plain_ms / kms,
-           "images_identical": same, "rse": rse_stats(jobs)}
+           "images_identical": same, "rse": rse_stats(jobs),
+           "plain_reference_lanes": {"lanes": M.NUM_THREADS, "photons_per_s": total / lanes_wall,
+                                     "device_ms": lanes_ms, "speedup_wall": lanes_wall / wall_b,
+                                     "rse": rse_stats(lane_res)}}
     b.close()
     return out
 
@@ -178,6 +200,7 @@ def bench_serve(ctx, rate=3.0, duration=12.0, epoch=4_000_000):
             lat[rid] = tnow - arr[rid][0]
             srv.collect(rid, arr[rid][1])
     span_s = time.perf_counter() - t0
+    cnt = srv.counters()
     srv.close()
     served = np.array(sorted(lat.values()))
     print(f"served: {len(served)} requests, span {span_s:.1f} s", file=sys.stderr, flush=True)
@@ -205,13 +228,45 @@ def bench_serve(ctx, rate=3.0, duration=12.0, epoch=4_000_000):
     span_p = time.perf_counter() - t0
     plain = np.array(sorted(plat))
     print(f"plain: span {span_p:.1f} s", file=sys.stderr, flush=True)
+
+    # plain loop with the reference's launch geometry (32768 MWC lanes per simulation)
+    t0 = time.perf_counter()
+    llat = []
+    for k, (ta, m, src, n, seed, fl) in enumerate(reqs):
+        now = time.perf_counter() - t0
+        if now < ta:
+            time.sleep(ta - now)
+        s = media[m]
+        s.s.number_of_photons = max(n, M.NUM_THREADS)
+        s.s.xi, s.s.yi, s.s.zi = src
+        s.s.src_kind = M.SOURCE_PENCIL
+        s.s.fhd_activated = 1 if fl else 0
+        x, a = M.init_rng(M.NUM_THREADS, seed=seed)
+        r = ctx.run_excitation(s, lanes=(x, a))
+        if fl:
+            Fx = r.fhd.astype(np.float64) / (4294967295.0 * r.totals["photons"])
+            job = M.BatchJob(0, n, seed, src_kind=M.SOURCE_EMISSION, Fx=Fx)
+            bb = M.Batch(ctx, [s], [job])
+            bb.run()
+            bb.close()
+        llat.append(time.perf_counter() - t0 - ta)
+        if k % 8 == 7:
+            print(f"plain, reference lanes: {k + 1} requests", file=sys.stderr, flush=True)
+    span_l = time.perf_counter() - t0
+    lanes = np.array(sorted(llat))
+    print(f"plain, reference lanes: span {span_l:.1f} s", file=sys.stderr, flush=True)
     return {"workload": f"Poisson arrivals {rate}/s for {duration}s ({len(reqs)} requests), 8 media up to "
                         f"128x128x64, scan sources, 50% with fluorescence, budgets log-uniform [1e5,1e8]",
             "photons": int(work), "epoch_photons": epoch,
             "served": {"p50_s": float(np.percentile(served, 50)), "p99_s": float(np.percentile(served, 99)),
-                       "photons_per_s": work / span_s, "span_s": span_s},
+                       "photons_per_s": work / span_s, "span_s": span_s, "epochs": cnt["epochs"],
+                       "device_photons_per_s": cnt["photons"] / max(cnt["kernel_ms"], 1e-9) * 1e3,
+                       "kernel_s": cnt["kernel_ms"] / 1e3, "host_s": cnt["host_ms"] / 1e3},
             "plain": {"p50_s": float(np.percentile(plain, 50)), "p99_s": float(np.percentile(plain, 99)),
-                      "photons_per_s": work / span_p, "span_s": span_p}}
+                      "photons_per_s": work / span_p, "span_s": span_p},
+            "plain_reference_lanes": {"p50_s": float(np.percentile(lanes, 50)),
+                                      "p99_s": float(np.percentile(lanes, 99)),
+                                      "photons_per_s": work / span_l, "span_s": span_l}}
 
 
 def main():

@@ -404,6 +404,8 @@ def _bind_engine(L):
                                    C.POINTER(C.c_double)]
     L.mcfl_server_collect.argtypes = [C.c_void_p, C.c_int64, C.POINTER(RequestResult)]
     L.mcfl_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
+    L.mcfl_server_counters.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mcfl_server_destroy.argtypes = [C.c_void_p]
     L.mcfl_server_destroy.restype = None
     L._engine = True
@@ -411,7 +413,7 @@ def _bind_engine(L):
 
 EXPORTS += ["mcfl_batch_create", "mcfl_batch_run", "mcfl_batch_destroy", "mcfl_run_batch", "mcfl_server_create",
             "mcfl_server_add_medium", "mcfl_server_submit", "mcfl_server_step", "mcfl_server_collect",
-            "mcfl_server_stats", "mcfl_server_destroy"]
+            "mcfl_server_stats", "mcfl_server_counters", "mcfl_server_destroy"]
 
 
 class BatchJob:
@@ -531,6 +533,12 @@ class Server:
         q, a, b = C.c_int32(), C.c_int32(), C.c_uint64()
         check(lib().mcfl_server_stats(self.h, C.byref(q), C.byref(a), C.byref(b)), "stats")
         return {"queued": q.value, "active": a.value, "bytes": b.value}
+
+    def counters(self):
+        """Cumulative photons launched, epochs, kernel ms and host ms spent in steps."""
+        p, e, k, h = C.c_uint64(), C.c_uint64(), C.c_double(), C.c_double()
+        check(lib().mcfl_server_counters(self.h, C.byref(p), C.byref(e), C.byref(k), C.byref(h)), "counters")
+        return {"photons": p.value, "epochs": e.value, "kernel_ms": k.value, "host_ms": h.value}
 
     def close(self):
         if self.h:

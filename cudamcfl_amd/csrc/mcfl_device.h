@@ -18,6 +18,16 @@
 #define MCFL_MAX_JOBS 64         /* explicit jobs per launch (per-job totals live in LDS) */
 #define MCFL_NTOT 16             /* totals per job, see enum mcfl_total in mcfl.h */
 #define MCFL_MAX_STEPS_PER_PHOTON (1u << 28)
+/* Transport workgroup: MCFL_WG threads (whole wavefronts), register
+   allocation for MCFL_WAVES_PER_EU resident waves per SIMD.  The LDS tile of
+   the photon hitting density is per workgroup, so larger workgroups at the
+   same occupancy share a larger tile. */
+#ifndef MCFL_WG
+#define MCFL_WG 768 /* 12 waves: 3 per SIMD, two workgroups (80 KB of LDS each) per CU */
+#endif
+#ifndef MCFL_WAVES_PER_EU
+#define MCFL_WAVES_PER_EU 6 /* 80 VGPRs */
+#endif
 #define MCFL_TALLY_COPIES 8      /* per-step volume tallies: one copy per XCD, summed after the launch */
 
 enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2 };

@@ -11,8 +11,10 @@
 // excitation, phase 1 emission -- whatever launch, epoch or lane runs it, so a
 // job's tallies equal those of the same job run alone.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -273,15 +275,14 @@ struct Launcher {
     release(d_counter);
   }
   mcfl_status prepare(mcfl_ctx* c, const std::vector<JobDev>& jd, const MatTable& mt, bool perjob,
-                      const Medium* uniform, LaunchArgs* A) {
+                      const Medium* uniform, LaunchArgs* A, hipStream_t st) {
     if (mt.mats.size() > MCFL_MAX_MATERIALS) return MCFL_ERR_CAPACITY;
     if (jd.size() > MCFL_MAX_JOBS) return MCFL_ERR_CAPACITY;
     MCFL_STCHK(ensure(d_jobs, jd.size() * sizeof(JobDev)));
     MCFL_STCHK(ensure(d_mats, mt.mats.size() * sizeof(MatDev)));
     MCFL_STCHK(ensure(d_counter, 8));
-    MCFL_HIPCHK(hipMemcpyAsync(d_jobs.p, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, c->stream));
-    MCFL_HIPCHK(hipMemcpyAsync(d_mats.p, mt.mats.data(), mt.mats.size() * sizeof(MatDev), hipMemcpyHostToDevice,
-                               c->stream));
+    MCFL_HIPCHK(hipMemcpyAsync(d_jobs.p, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
+    MCFL_HIPCHK(hipMemcpyAsync(d_mats.p, mt.mats.data(), mt.mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, st));
     memset(A, 0, sizeof *A);
     if (!perjob) fill_medium(&uniform->sim, uniform->d_labels.as<const uint8_t>(), &A->med);
     A->mats = d_mats.as<const MatDev>();
@@ -361,7 +362,7 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
       uni = b->media[i].get();
     }
   b->perjob = n_used > 1;
-  MCFL_STCHK(b->L.prepare(c, b->jd, b->mt, b->perjob, uni, &b->A));
+  MCFL_STCHK(b->L.prepare(c, b->jd, b->mt, b->perjob, uni, &b->A, c->stream));
   for (int j = 0; j < n_jobs; ++j)  // LDS fhd tile for the first job tallying fhd
     if (b->jd[j].fhd != nullptr) {
       const Medium& m = *b->media[jobs[j].medium];
@@ -371,7 +372,7 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
       break;
     }
   b->nblocks = grid_blocks(c, b->A, b->temporal, b->perjob);
-  b->A.n_waves = (uint32_t)b->nblocks * 4u;
+  b->A.n_waves = (uint32_t)b->nblocks * (MCFL_WG / 64u);
   MCFL_HIPCHK(hipStreamSynchronize(c->stream));
 
   // Capture: clear every tally and the work counter, then the transport launch.
@@ -448,8 +449,9 @@ namespace {
 struct Request {
   int64_t id = 0;
   mcfl_request r{};
-  int phase = 0;  // 0 queued, 1 excitation, 2 emission, 3 done
-  uint64_t done = 0;
+  int phase = 0;       // 0 queued, 1 excitation, 2 emission, 3 done
+  uint64_t done = 0;   // photons of the current phase handed to epochs
+  uint64_t finished = 0;  // photons of the current phase whose epochs completed
   uint64_t epochs = 0;
   size_t footprint = 0;
   Medium* m = nullptr;
@@ -467,15 +469,42 @@ size_t footprint_of(const Medium& m, const mcfl_request& r, uint32_t copies) {
 
 }  // namespace
 
+// One launch of the server: its own job table, counter and stream, and the
+// request slices it carries.
+struct Epoch {
+  Launcher L;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool busy = false;
+  std::vector<std::pair<int64_t, uint64_t>> parts;  // request id, photons
+  uint64_t photons = 0;
+};
+
 struct mcfl_server {
   mcfl_ctx* c = nullptr;
   uint64_t budget = 0, epoch_photons = 0, in_use = 0;
+  uint64_t epoch_target = 10000000;  // photons per epoch to aim for (env MCFL_EPOCH_TARGET)
   std::vector<std::unique_ptr<Medium>> media;
   std::deque<std::unique_ptr<Request>> queue;
   std::vector<std::unique_ptr<Request>> active;
   std::map<int64_t, std::unique_ptr<Request>> done;
   int64_t next_id = 1;
-  Launcher L;
+  // Two epochs in flight on two streams: the workgroups of the newer launch
+  // take the CUs that the older one's last photons leave idle.
+  Epoch ep[2];
+  int next = 0;              // slot of the next launch
+  std::deque<int> inflight;  // slots in launch order
+  // cumulative counters
+  uint64_t photons = 0, epochs = 0;
+  double kernel_ms = 0.0, host_ms = 0.0;
+  ~mcfl_server() {
+    for (Epoch& e : ep) {
+      if (e.stream) (void)hipStreamSynchronize(e.stream);
+      if (e.e0) (void)hipEventDestroy(e.e0);
+      if (e.e1) (void)hipEventDestroy(e.e1);
+      if (e.stream) (void)hipStreamDestroy(e.stream);
+    }
+  }
 };
 
 extern "C" mcfl_status mcfl_server_create(mcfl_ctx* c, uint64_t budget, uint64_t epoch_photons, mcfl_server** out) {
@@ -485,6 +514,17 @@ extern "C" mcfl_status mcfl_server_create(mcfl_ctx* c, uint64_t budget, uint64_t
   s->c = c;
   s->budget = budget;
   s->epoch_photons = epoch_photons;
+  if (const char* e = getenv("MCFL_EPOCH_TARGET")) s->epoch_target = strtoull(e, nullptr, 10);
+  if (hipSetDevice(c->device) != hipSuccess) {
+    delete s;
+    return MCFL_ERR_DEVICE;
+  }
+  for (Epoch& e : s->ep)
+    if (hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e.e0) != hipSuccess ||
+        hipEventCreate(&e.e1) != hipSuccess) {
+      delete s;
+      return MCFL_ERR_DEVICE;
+    }
   *out = s;
   return MCFL_OK;
 }
@@ -524,12 +564,14 @@ mcfl_status admit(mcfl_server* s) {
     Request& q = *s->queue.front();
     if (s->in_use + q.footprint > s->budget) break;  // admission by memory footprint, FIFO
     q.ex.reset(new Tallies());
+    // cleared on the stream of the next launch, the first one that can carry the request
+    const hipStream_t st = s->ep[s->next].stream;
     MCFL_STCHK(alloc_tallies(*q.m, q.r.fluorescence ? (uint32_t)MCFL_TALLY_FHD : 0u, false, s->c->copies, q.ex.get()));
-    MCFL_STCHK(clear_tallies(*q.m, q.ex.get(), s->c->stream));
+    MCFL_STCHK(clear_tallies(*q.m, q.ex.get(), st));
     if (q.r.fluorescence) {
       q.em.reset(new Tallies());
       MCFL_STCHK(alloc_tallies(*q.m, 0u, true, s->c->copies, q.em.get()));
-      MCFL_STCHK(clear_tallies(*q.m, q.em.get(), s->c->stream));
+      MCFL_STCHK(clear_tallies(*q.m, q.em.get(), st));
     }
     q.phase = 1;
     s->in_use += q.footprint;
@@ -540,15 +582,15 @@ mcfl_status admit(mcfl_server* s) {
 }
 
 // Phase finished: start the emission pass or retire the request.
-mcfl_status advance(mcfl_server* s, Request& q) {
+mcfl_status advance(mcfl_server* s, Request& q, hipStream_t st) {
   const Medium& m = *q.m;
   if (q.phase == 1) {
     q.Rd.resize(m.npix);
     q.Tt.resize(m.npix);
     MCFL_STCHK(download(m, *q.ex, q.Rd.data(), q.Tt.data(), nullptr, nullptr, nullptr, nullptr, nullptr, q.tot));
     if (q.r.fluorescence) {
-      if (reduce_tallies(m, q.ex.get(), s->c->stream) != 0) return MCFL_ERR_DEVICE;
-      MCFL_HIPCHK(hipStreamSynchronize(s->c->stream));
+      if (reduce_tallies(m, q.ex.get(), st) != 0) return MCFL_ERR_DEVICE;
+      MCFL_HIPCHK(hipStreamSynchronize(st));
       std::vector<uint64_t> fhd(m.nvox);
       MCFL_HIPCHK(hipMemcpy(fhd.data(), q.ex->fhd.p, m.nvox * 8, hipMemcpyDeviceToHost));
       std::vector<double> Fx(m.nvox);
@@ -558,6 +600,7 @@ mcfl_status advance(mcfl_server* s, Request& q) {
       q.fl_unit = q.em->fl_unit;
       q.phase = 2;
       q.done = 0;
+      q.finished = 0;
       if (q.em->al_n > 0) return MCFL_OK;
     }
   }
@@ -582,69 +625,118 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
   if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
   if (n_done) *n_done = 0;
   if (kernel_ms) *kernel_ms = 0.0;
+  const auto t_host0 = std::chrono::steady_clock::now();
+  double t_wait = 0.0;
   MCFL_STCHK(admit(s));
-  if (s->active.empty()) return MCFL_OK;
 
-  // Water-filling share of the epoch over the active requests.
-  const size_t n = s->active.size();
-  std::vector<uint64_t> rem(n), take(n, 0);
-  for (size_t i = 0; i < n; ++i) rem[i] = s->active[i]->r.photons - s->active[i]->done;
-  uint64_t left = s->epoch_photons;
-  size_t open = n;
-  while (left > 0 && open > 0) {
-    const uint64_t share = std::max<uint64_t>(1, left / open);
-    open = 0;
-    for (size_t i = 0; i < n && left > 0; ++i) {
-      const uint64_t want = rem[i] - take[i];
-      if (want == 0) continue;
-      const uint64_t t = std::min({want, share, left});
-      take[i] += t;
-      left -= t;
-      if (take[i] < rem[i]) ++open;
+  // 1. Launch the next epoch if its slot is free.  Every active request with
+  // photons left in its phase advances by an equal quantum: the epoch target
+  // (photons of about 25 ms of device time) split over them, at least
+  // epoch_photons.  A launch is therefore large enough to fill the GPU, a lone
+  // large request finishes in a few epochs, and each request gets an equal
+  // share of the device.
+  bool launched = false;
+  Epoch& E = s->ep[s->next];
+  size_t n_open = 0;
+  for (const std::unique_ptr<Request>& qp : s->active)
+    if ((qp->phase == 1 || qp->phase == 2) && qp->done < qp->r.photons) ++n_open;
+  // per-request share of the epoch target, never below epoch_photons
+  const uint64_t quantum = std::max<uint64_t>(s->epoch_photons, n_open ? s->epoch_target / n_open : 0);
+  if (!E.busy) {
+    std::vector<JobDev> jd;
+    MatTable mt;
+    E.parts.clear();
+    const Medium* uni = nullptr;
+    bool perjob = false;
+    uint64_t begin = 0;
+    int tile_job = -1;
+    for (std::unique_ptr<Request>& qp : s->active) {
+      Request& q = *qp;
+      if ((q.phase != 1 && q.phase != 2) || q.done >= q.r.photons) continue;
+      const uint64_t take = std::min(q.r.photons - q.done, quantum);
+      if (!uni) uni = q.m;
+      if (q.m != uni) perjob = true;
+      const bool em = q.phase == 2;
+      Tallies& t = em ? *q.em : *q.ex;
+      JobDev J;
+      fill_job(J, *q.m, t, em ? MCFL_SOURCE_EMISSION : q.r.src_kind, q.r.xi, q.r.yi, q.r.zi, q.r.src_r, q.r.seed,
+               begin, take, q.done, mt.base_of(q.m, em));
+      if (J.fhd != nullptr && tile_job < 0) tile_job = (int)jd.size();
+      jd.push_back(J);
+      E.parts.emplace_back(q.id, take);
+      begin += take;
+      q.done += take;
+      ++q.epochs;
+    }
+    if (!jd.empty()) {
+      LaunchArgs A;
+      MCFL_STCHK(E.L.prepare(c, jd, mt, perjob, uni, &A, E.stream));
+      if (tile_job >= 0) {  // LDS fhd tile for the first request in its excitation phase
+        const Request* q = nullptr;
+        for (const std::unique_ptr<Request>& qp : s->active)
+          if (qp->id == E.parts[tile_job].first) q = qp.get();
+        MediumDev md;
+        fill_medium(&q->m->sim, q->m->d_labels.as<const uint8_t>(), &md);
+        choose_tile(c, A, md, q->r.xi, q->r.yi, (uint32_t)tile_job, 0, perjob ? 1 : 0);
+      }
+      const int nblocks = grid_blocks(c, A, 0, perjob ? 1 : 0);
+      A.n_waves = (uint32_t)nblocks * (MCFL_WG / 64u);
+      MCFL_HIPCHK(hipMemsetAsync(A.counter, 0, 8, E.stream));
+      MCFL_HIPCHK(hipEventRecord(E.e0, E.stream));
+      if (mcfl_transport_launch(&A, 0, perjob ? 1 : 0, nblocks, E.stream) != 0) return MCFL_ERR_DEVICE;
+      MCFL_HIPCHK(hipEventRecord(E.e1, E.stream));
+      E.photons = A.total;
+      E.busy = true;
+      s->inflight.push_back(s->next);
+      s->next ^= 1;
+      launched = true;
     }
   }
 
-  std::vector<JobDev> jd;
-  MatTable mt;
-  const Medium* uni = s->active[0]->m;
-  bool perjob = false;
-  uint64_t begin = 0;
-  for (size_t i = 0; i < n; ++i) {
-    Request& q = *s->active[i];
-    if (q.m != uni) perjob = true;
-    const bool em = q.phase == 2;
-    Tallies& t = em ? *q.em : *q.ex;
-    JobDev J;
-    fill_job(J, *q.m, t, em ? MCFL_SOURCE_EMISSION : q.r.src_kind, q.r.xi, q.r.yi, q.r.zi, q.r.src_r, q.r.seed,
-             begin, take[i], q.done, mt.base_of(q.m, em));
-    jd.push_back(J);
-    begin += take[i];
-    if (take[i] > 0) ++q.epochs;
-  }
-  LaunchArgs A;
-  MCFL_STCHK(s->L.prepare(c, jd, mt, perjob, uni, &A));
-  MCFL_HIPCHK(hipMemsetAsync(A.counter, 0, 8, c->stream));
-  double ms = 0.0;
-  if (A.total > 0) MCFL_STCHK(launch_and_time(c, A, 0, perjob ? 1 : 0, &ms));
-  MCFL_HIPCHK(hipStreamSynchronize(c->stream));
-  if (kernel_ms) *kernel_ms = ms;
-
+  // 2. Complete the oldest epoch when both slots are busy or nothing new
+  // could start, then move finished phases on and retire finished requests.
   int32_t nd = 0;
-  std::vector<std::unique_ptr<Request>> still;
-  for (size_t i = 0; i < n; ++i) {
-    std::unique_ptr<Request>& q = s->active[i];
-    q->done += take[i];
-    if (q->done >= q->r.photons) MCFL_STCHK(advance(s, *q));
-    if (q->phase == 3) {
-      if (done_ids && nd < max_done) done_ids[nd] = q->id;
-      ++nd;
-      s->done[q->id] = std::move(q);
-    } else {
-      still.push_back(std::move(q));
+  if (!s->inflight.empty() && (s->inflight.size() == 2 || !launched)) {
+    Epoch& O = s->ep[s->inflight.front()];
+    s->inflight.pop_front();
+    const auto tw0 = std::chrono::steady_clock::now();
+    MCFL_HIPCHK(hipEventSynchronize(O.e1));
+    t_wait = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw0).count();
+    float ms = 0.f;
+    MCFL_HIPCHK(hipEventElapsedTime(&ms, O.e0, O.e1));
+    if (kernel_ms) *kernel_ms = ms;
+    s->photons += O.photons;
+    s->epochs += 1;
+    s->kernel_ms += ms;
+    O.busy = false;
+    for (const auto& pt : O.parts)
+      for (std::unique_ptr<Request>& qp : s->active)
+        if (qp->id == pt.first) qp->finished += pt.second;
+    std::vector<std::unique_ptr<Request>> still;
+    for (std::unique_ptr<Request>& q : s->active) {
+      if ((q->phase == 1 || q->phase == 2) && q->finished >= q->r.photons) MCFL_STCHK(advance(s, *q, O.stream));
+      if (q->phase == 3) {
+        if (done_ids && nd < max_done) done_ids[nd] = q->id;
+        ++nd;
+        s->done[q->id] = std::move(q);
+      } else {
+        still.push_back(std::move(q));
+      }
     }
+    s->active.swap(still);
   }
-  s->active.swap(still);
   if (n_done) *n_done = nd;
+  s->host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count() - t_wait;
+  return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_server_counters(mcfl_server* s, uint64_t* photons, uint64_t* epochs, double* kernel_ms,
+                                            double* host_ms) {
+  if (!s) return MCFL_ERR_ARG;
+  if (photons) *photons = s->photons;
+  if (epochs) *epochs = s->epochs;
+  if (kernel_ms) *kernel_ms = s->kernel_ms;
+  if (host_ms) *host_ms = s->host_ms;
   return MCFL_OK;
 }
 
@@ -678,5 +770,5 @@ extern "C" void mcfl_server_destroy(mcfl_server* s) {
   if (!s) return;
   (void)hipSetDevice(s->c->device);
   (void)hipStreamSynchronize(s->c->stream);
-  delete s;
+  delete s;  // waits for both epoch streams
 }

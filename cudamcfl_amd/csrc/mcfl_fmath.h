@@ -20,7 +20,9 @@
  *    Max relative error about 2 ulp on (0, 1].
  *  - reciprocal square root: integer-shift seed with Lomont's constant
  *    0x5F375A86 (seed error up to 3.4%) and three Newton-Raphson steps
- *    written with FMA; about 1 ulp.
+ *    written with FMA; about 1 ulp.  Square roots on the per-step path are
+ *    x * rsqrt(x); renormalisation factors of near-unit vectors use the
+ *    first-order expansion of 1/sqrt around 1.
  *  - sine/cosine of 2*pi*t ("turns"): exact quadrant reduction 4t - rint(4t)
  *    (no large-argument reduction is needed because the argument is a
  *    fraction of a turn), then odd/even degree-9/8 polynomials in the
@@ -103,6 +105,16 @@ MCFL_HD float mcfl_rsqrt(float x) {
   y = mcfl_fma(y, e, y);
   e = mcfl_fma(-h * y, y, 0.5f);
   return mcfl_fma(y, e, y);
+}
+
+/* sqrt(x) for x = 0 or a positive normal x, as x * rsqrt(x): within 2 ulp. */
+MCFL_HD float mcfl_sqrt_rs(float x) { return x > 0.0f ? x * mcfl_rsqrt(x) : 0.0f; }
+
+/* 1/sqrt(x) as a renormalisation factor: for |x - 1| < 2^-12 the first-order
+   expansion 1.5 - x/2 (one FMA) is off by 3/8 (x-1)^2 < 2^-25, below half an
+   ulp of the result; farther from 1 the full reciprocal root. */
+MCFL_HD float mcfl_renorm(float x) {
+  return mcfl_fabs(x - 1.0f) < 0x1p-12f ? mcfl_fma(x, -0.5f, 1.5f) : mcfl_rsqrt(x);
 }
 
 /* sin(2*pi*t) and cos(2*pi*t). */

@@ -155,8 +155,11 @@ void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi,
   const size_t per_wg = c->lds_per_cu / (size_t)bpc;
   const size_t other = mcfl_lds_bytes(nslots, 0, A.n_mats);
   if (per_wg <= other + 64 * 8) return;
-  uint32_t cap = (uint32_t)std::min<size_t>((per_wg - other) / 8, 4096);
-  const uint32_t tnx = (uint32_t)std::min(m.nx, 16), tny = (uint32_t)std::min(m.ny, 16);
+  const uint32_t cap = (uint32_t)std::min<size_t>((per_wg - other) / 8, 1u << 15);
+  // box 2r x 2r x r around the source (the photon cloud is about as deep as it is wide)
+  int side = std::max(2, (int)std::lround(2.0 * std::cbrt(cap / 4.0)));
+  if (c->tile_side > 0) side = std::max(2, std::min(c->tile_side, (int)std::sqrt((double)cap)));
+  const uint32_t tnx = (uint32_t)std::min(m.nx, side), tny = (uint32_t)std::min(m.ny, side);
   const uint32_t tnz = std::min<uint32_t>((uint32_t)m.nz, cap / (tnx * tny));
   if (tnz == 0) return;
   auto clampi = [](long v, long lo, long hi) { return v < lo ? lo : (v > hi ? hi : v); };
@@ -187,7 +190,7 @@ void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi,
 mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms,
                             const std::function<int()>& post) {
   const int nblocks = grid_blocks(c, A, temporal, perjob);
-  A.n_waves = (uint32_t)nblocks * 4u;
+  A.n_waves = (uint32_t)nblocks * (MCFL_WG / 64u);
   if (A.chunk_max == 0) A.chunk_max = 1024;
   A.refill_min = c->refill_min;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -333,6 +336,7 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   c->n_cu = prop.multiProcessorCount;
   if (const char* e = getenv("MCFL_TALLY_COPIES")) c->copies = atoi(e) > 1 ? MCFL_TALLY_COPIES : 1;
+  if (const char* e = getenv("MCFL_TILE_SIDE")) c->tile_side = atoi(e);
   if (const char* e = getenv("MCFL_REFILL_MIN")) c->refill_min = (uint32_t)std::min(64, std::max(1, atoi(e)));
   snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
   for (int w = 0; w < 3; ++w)

@@ -54,6 +54,7 @@ struct mcfl_ctx {
   int bpc[3][2][2] = {};  // resident workgroups per CU [work][temporal][perjob], register-limited
   size_t lds_per_cu = 160 * 1024;
   uint32_t copies = MCFL_TALLY_COPIES;  // copies of the per-step volume tallies (1 or MCFL_TALLY_COPIES)
+  int tile_side = 0;                    // lateral size of the fhd LDS tile, 0: from the LDS budget (env MCFL_TILE_SIDE)
   uint32_t refill_min = 4;              // idle lanes that trigger a wave's refill (env MCFL_REFILL_MIN)
   char name[256] = {};
   uint32_t* d_atab = nullptr;

@@ -32,7 +32,6 @@
 #include "mcfl_device.h"
 #include "mcfl_mwc.h"
 
-#define MCFL_WG 256
 
 namespace {
 
@@ -137,7 +136,7 @@ __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint3
   float t = mt.onemg2 / mcfl_fma(mt.twog, xi, mt.onemg);
   float cost = mcfl_fma(-t, t, mt.onepg2) * mt.inv2g;
   if (mt.twog == 0.0f) cost = 2.0f * mcfl_rand_co(&x, a) - 1.0f;
-  float sint = mcfl_sqrt(mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)));
+  float sint = mcfl_sqrt_rs(mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f))); /* 0 or >= 2^-23 */
   float sps, cps;
   mcfl_sincos_turn(mcfl_rand_co(&x, a), &sps, &cps);
   const float cosp = sps, sinp = cps; /* the reference's __sincosf argument order */
@@ -156,7 +155,7 @@ __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint3
     ny = mcfl_fma(stt, dy * dz * cosp + dx * sinp, dy * cost);
     nz = mcfl_fma(-sint * cosp, tt, dz * cost);
   }
-  const float r = mcfl_rsqrt(nx * nx + ny * ny + nz * nz);
+  const float r = mcfl_renorm(nx * nx + ny * ny + nz * nz);
   p.dx = nx * r;
   p.dy = ny * r;
   p.dz = nz * r;
@@ -312,7 +311,7 @@ __device__ __forceinline__ uint32_t find_job(const LaunchArgs& A, uint64_t g, ui
 } /* namespace */
 
 template <int WORK, bool TEMPORAL, bool PERJOB>
-__global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
+__global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WAVES_PER_EU))) void mcfl_transport_kernel(LaunchArgs A) {
   /* dynamic LDS: [per-job totals | fhd tile | materials] */
   extern __shared__ __align__(16) unsigned long long lds[];
 
@@ -465,14 +464,10 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
     unsigned long long* st = stot + job * MCFL_NTOT;
     const MatDev mc = mats[p.lab];
     float s;
-    uint32_t nb;
-    bool in;
-    uint32_t vi;
-    Cell cl;
+    uint32_t nb = p.lab;
     ++iters;
     if (mc.mutr != kFltMax) {
       s = -mcfl_log(mcfl_rand_oc(&x, a)) * mc.mutr;
-      nb = p.lab;
       if (mcfl_fma(s, p.dz, p.z) < 0.0f) {
         nb = 0;
         s = -p.z / p.dz;
@@ -485,17 +480,14 @@ __global__ __launch_bounds__(MCFL_WG) void mcfl_transport_kernel(LaunchArgs A) {
       p.y = mcfl_fma(p.dy, s, p.y);
       p.z = mcfl_fma(p.dz, s, p.z);
       if (TEMPORAL) p.tof += mcfl_d2u64_rz((double)(s * mc.n) / kCcmfs);
-      in = in_grid(m, p.x, p.y, p.z);
-      cl = cell_in(m, p.x, p.y, p.z);
-      vi = in ? cell_index(m, cl) : 0u;
-      if (nb != 0u && nb != m.last) nb = in ? (uint32_t)m.labels[vi] : 1u;
     } else {
       s = 100.0f;
       nb = glass_march<TEMPORAL>(m, p, p.lab, s);
-      in = in_grid(m, p.x, p.y, p.z);
-      cl = cell_in(m, p.x, p.y, p.z);
-      vi = in ? cell_index(m, cl) : 0u;
     }
+    const bool in = in_grid(m, p.x, p.y, p.z);
+    const Cell cl = cell_in(m, p.x, p.y, p.z);
+    const uint32_t vi = in ? cell_index(m, cl) : 0u;
+    if (nb == p.lab && mc.mutr != kFltMax) nb = in ? (uint32_t)m.labels[vi] : 1u; /* :319-333 */
     MatDev mm = mc; /* material after a possible interface crossing */
     if (nb != p.lab) {
       reflect(mats, p, nb, x, a);

@@ -291,16 +291,23 @@ typedef struct mcfl_request_result {
   uint64_t epochs; /* epochs the request took part in */
 } mcfl_request_result;
 
+/* Each epoch advances every active request by an equal quantum in one launch:
+   an epoch target of 1e7 photons split over them, at least epoch_photons.
+   Admission is FIFO within mem_budget_bytes.  Two epochs are in flight at a
+   time on two streams. */
 mcfl_status mcfl_server_create(mcfl_ctx* ctx, uint64_t mem_budget_bytes, uint64_t epoch_photons, mcfl_server** out);
 /* Makes a medium resident; the simulation's bulks/labels/detector are copied. */
 mcfl_status mcfl_server_add_medium(mcfl_server* s, const mcfl_sim* medium, int32_t* medium_id);
 /* Queues a request (admitted at the next step that has room). */
 mcfl_status mcfl_server_submit(mcfl_server* s, const mcfl_request* req, int64_t* request_id);
-/* Admits, runs one epoch, retires finished requests (their ids in done_ids). */
+/* Admits, launches the next epoch, completes the oldest one in flight and
+   retires finished requests (their ids in done_ids). */
 mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32_t max_done, int32_t* n_done, double* kernel_ms);
 /* Copies a finished request's results and forgets it. */
 mcfl_status mcfl_server_collect(mcfl_server* s, int64_t request_id, mcfl_request_result* out);
 mcfl_status mcfl_server_stats(mcfl_server* s, int32_t* queued, int32_t* active, uint64_t* bytes_in_use);
+/* Cumulative photons launched, epochs, kernel time and host time spent in steps outside the kernel. */
+mcfl_status mcfl_server_counters(mcfl_server* s, uint64_t* photons, uint64_t* epochs, double* kernel_ms, double* host_ms);
 void mcfl_server_destroy(mcfl_server* s);
 
 /* ---- writers (byte layouts of the reference) ---- */

@@ -170,7 +170,7 @@ static void orc_spin(const mat_t* m, orc_photon* p, uint64_t* x, uint32_t a) {
   float temp = m->onemg2 / mcfl_fma(m->twog, xi, m->onemg);
   float cost = mcfl_fma(-temp, temp, m->onepg2) * m->inv2g;
   if (m->g == 0.0f) cost = 2.0f * mcfl_rand_co(x, a) - 1.0f;
-  float sint = mcfl_sqrt(mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)));
+  float sint = mcfl_sqrt_rs(mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)));
   float s_psi, c_psi;
   mcfl_sincos_turn(mcfl_rand_co(x, a), &s_psi, &c_psi);
   float cosp = s_psi; /* :603 assigns the sine to cosp */
@@ -191,8 +191,8 @@ static void orc_spin(const mat_t* m, orc_photon* p, uint64_t* x, uint32_t a) {
     ndy = mcfl_fma(st, dy * dz * cosp + dx * sinp, dy * cost);
     ndz = mcfl_fma(-sint * cosp, tt, dz * cost);
   }
-  /* :621-624 renormalisation with a reciprocal square root */
-  float r = mcfl_rsqrt(ndx * ndx + ndy * ndy + ndz * ndz);
+  /* :621-624 renormalisation with a reciprocal square root (first order near 1) */
+  float r = mcfl_renorm(ndx * ndx + ndy * ndy + ndz * ndz);
   p->dx = ndx * r;
   p->dy = ndy * r;
   p->dz = ndz * r;
@@ -723,6 +723,8 @@ void orc_math_eval(int fn, const float* in, float* out, size_t n) {
       case 2: mcfl_sincos_turn(in[i], &s, &c); out[i] = c; break;
       case 3: { uint32_t u; memcpy(&u, &in[i], 4); out[i] = mcfl_u2f_rz(u); } break;
       case 4: out[i] = mcfl_rsqrt(in[i]); break;
+      case 5: out[i] = mcfl_sqrt_rs(in[i]); break;
+      case 6: out[i] = mcfl_renorm(in[i]); break;
       default: out[i] = 0.0f;
     }
   }

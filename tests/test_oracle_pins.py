@@ -63,6 +63,24 @@ def test_rsqrt_error_bound():
     assert np.max(np.abs(got - ref) / ulp) <= 2.0
 
 
+def test_sqrt_and_renormalisation_error_bounds():
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(1.2e-7, 1.0, 300000), np.geomspace(1.2e-38, 1e30, 20000)]).astype(np.float32)
+    got = O.math_eval(5, x).astype(np.float64)
+    ref = np.sqrt(x.astype(np.float64))
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(got - ref) / ulp) <= 2.0
+    assert O.math_eval(5, np.zeros(1, np.float32))[0] == 0.0
+    # near 1 the one-FMA expansion, elsewhere the full root: both within 1 ulp
+    y = np.concatenate([1 + rng.uniform(-2 ** -12, 2 ** -12, 200000), rng.uniform(0.5, 2.0, 50000)]).astype(np.float32)
+    got = O.math_eval(6, y).astype(np.float64)
+    ref = 1.0 / np.sqrt(y.astype(np.float64))
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(got - ref) / ulp) <= 2.0
+    near = np.abs(y.astype(np.float64) - 1) < 2 ** -12
+    assert np.max(np.abs(got - ref)[near] / ulp[near]) <= 1.0
+
+
 def test_u32_to_float_toward_zero():
     rng = np.random.default_rng(2)
     u = np.concatenate([rng.integers(0, 2 ** 32, 100000, dtype=np.uint64),
