@@ -55,6 +55,19 @@ def cpu_baseline(target_s=12.0):
             "photon_steps_per_s": r.tot["steps"] / dt}
 
 
+def measured_traffic(photons):
+    """HBM bytes per launch of this workload from the committed counter
+    profile (FETCH_SIZE and WRITE_SIZE, each scaled on a plain copy); None when
+    the profile covers another photon count."""
+    path = os.path.join(ROOT, "profiles", "mcfl_transport_kernel_cfg2_1e8_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t["bytes_per_launch"] if t.get("photons") == photons else None
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size: the calibration point of
     the FETCH_SIZE / WRITE_SIZE counters."""
@@ -337,8 +350,9 @@ def main():
         "photon_steps_per_s": steps_per_s,
         "steps_per_photon": steps / (N * args.steps),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tslot/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "note": f"{F_STEP} VALU issue slots per photon-step x photon-steps / kernel time"},
+                     "frac": achieved / peak, "traffic": measured_traffic(N),
+                     "note": f"{F_STEP} VALU issue slots per photon-step x photon-steps / kernel time; traffic = "
+                             "HBM bytes per launch from FETCH_SIZE/WRITE_SIZE scaled on a copy (profiles/)"},
         "gpu": info,
     }
     if args.scan:
