@@ -31,10 +31,10 @@ def cpu_baseline(target_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import cudamcfl_amd as M
     import _oracle as O
-    from cudamcfl_amd.media import cfg2_simulation
+    from cudamcfl_amd.media import inclusion_simulation
 
     threads = min(16, os.cpu_count() or 1)
-    sim = cfg2_simulation(photons=10_000)
+    sim = inclusion_simulation(photons=10_000)
     o = O.FromProduct(sim)
     a_tab = M.multipliers(32768)
     key = M.stream_key(99, 0)
@@ -59,7 +59,7 @@ def measured_traffic(photons):
     """HBM bytes per launch of this workload from the committed counter
     profile (FETCH_SIZE and WRITE_SIZE, each scaled on a plain copy); None when
     the profile covers another photon count."""
-    path = os.path.join(ROOT, "profiles", "mcfl_transport_kernel_cfg2_1e8_traffic.json")
+    path = os.path.join(ROOT, "profiles", "mcfl_transport_kernel_128x128x64_1e8_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
@@ -103,8 +103,8 @@ def bench_scan(ctx):
     launch vs 64 plain launches with the same budgets and seeds (bit-identical
     images, hence identical per-pixel RSE)."""
     import cudamcfl_amd as M
-    from cudamcfl_amd.media import cfg2_simulation, scan_sources, scan_budgets
-    sim = cfg2_simulation(photons=1000, fhd=False)
+    from cudamcfl_amd.media import inclusion_simulation, scan_sources, scan_budgets
+    sim = inclusion_simulation(photons=1000, fhd=False)
     srcs, budgets = scan_sources(), scan_budgets()
     total = int(sum(budgets))
     jobs = [M.BatchJob(0, int(n), 700 + i, source=s, tallies=M.TALLY_SQ) for i, (s, n) in enumerate(zip(srcs, budgets))]
@@ -299,10 +299,10 @@ def main():
         sys.exit(2)
 
     import cudamcfl_amd as M
-    from cudamcfl_amd.media import cfg2_simulation
+    from cudamcfl_amd.media import inclusion_simulation
 
     N = int(args.photons)
-    sim = cfg2_simulation(photons=N)
+    sim = inclusion_simulation(photons=N)
     ctx = M.Context(0)
     if args.copy_probe:
         copy_probe(args.copy_probe)

@@ -49,7 +49,7 @@ def inclusions(shape, rng, n_spheres=5, r_sphere=(3, 8), n_cyl=2, r_cyl=(2, 5), 
     return lab
 
 
-def cfg2_medium(seed=2024, shape=(64, 128, 128)):
+def inclusion_medium(seed=2024, shape=(64, 128, 128)):
     """128x128x64 medium of 0.5 mm voxels: 5 spheres r in [3,8] voxels, 2
     z-parallel cylinders r in [2,5] voxels, 8 materials."""
     rng = np.random.default_rng(seed)
@@ -58,10 +58,10 @@ def cfg2_medium(seed=2024, shape=(64, 128, 128)):
     return lab, mats
 
 
-def cfg2_simulation(photons=100_000_000, seed=2024, fhd=True, source=(0.0, 0.0, 0.0)):
+def inclusion_simulation(photons=100_000_000, seed=2024, fhd=True, source=(0.0, 0.0, 0.0)):
     """The plain-path benchmark problem: grid_size 20 /cm (0.5 mm voxels),
     slab 3.2 cm, 128x128 whole-field images of 0.5 mm pixels, pencil beam."""
-    lab, mats = cfg2_medium(seed)
+    lab, mats = inclusion_medium(seed)
     nz, ny, nx = lab.shape
     gs = 20
     return Simulation.voxel(lab, mats, esp=nz / gs, grid_size=gs, det=(1.0 / gs, 1.0 / gs, nx, ny, 0.0, 0.0),

@@ -119,9 +119,9 @@ def test_parser_missing_file():
 
 
 def test_dims_extension_roundtrip(tmp_path, monkeypatch):
-    from cudamcfl_amd.media import write_bulk, write_mci, cfg2_medium
+    from cudamcfl_amd.media import write_bulk, write_mci, inclusion_medium
     monkeypatch.chdir(tmp_path)
-    lab, mats = cfg2_medium(seed=5, shape=(8, 12, 10))
+    lab, mats = inclusion_medium(seed=5, shape=(8, 12, 10))
     write_bulk("b.txt", lab)
     write_mci("m.mci", "out", "b.txt", mats, esp=0.4, grid_size=20, photons=10 ** 6, dims=(10, 12, 8))
     s = M.Simulation.from_mci("m.mci")
