@@ -25,7 +25,7 @@ F_STEP = 150            # VALU issue slots per photon-step of the reference algo
 PEAK_SLOTS = 256 * 4 * 16 * 2.4e9   # MI355X unpacked FP32 lane-issue rate (slots/s)
 
 
-def cpu_baseline(target_s=12.0):
+def cpu_baseline(target_s=20.0):
     """The CPU oracle on host cores, photon streams, same medium; a bounded
     sample sized to about target_s seconds."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -50,9 +50,16 @@ def cpu_baseline(target_s=12.0):
     t0 = time.perf_counter()
     r = O.run_photons(o.o, a_tab, key, 0, n, nthreads=threads)
     dt = time.perf_counter() - t0
+    # one host thread, a ~3 s sample
+    n1 = max(2000, int(n / threads * 3.0 / target_s))
+    t0 = time.perf_counter()
+    r1 = O.run_photons(o.o, a_tab, key, 0, n1, nthreads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": n / dt, "unit": "photons/s", "cores": threads, "kind": "port",
             "sample": f"{n} photons of the same 128x128x64 medium (photon streams), {dt:.1f} s",
-            "photon_steps_per_s": r.tot["steps"] / dt}
+            "photon_steps_per_s": r.tot["steps"] / dt,
+            "single_thread": {"photons_per_s": n1 / dt1, "photon_steps_per_s": r1.tot["steps"] / dt1,
+                              "sample": f"{n1} photons, {dt1:.1f} s"}}
 
 
 def measured_traffic(photons):

@@ -248,6 +248,78 @@ class Simulation:
             d.temp_bins, d.max_temp = bins, tmax
         return obj
 
+    @classmethod
+    def layered(cls, layers, inclusion, grid_size, det=(0.1, 0.1, 10, 10, 0.0, 0.0), photons=100000,
+                source=(0.0, 0.0, 0.0), src_kind=SOURCE_PENCIL, src_r=0.0, n_above=1.0, n_below=1.0, fhd=False,
+                temporal=None, outp="mcfl-out"):
+        """A bulk-method-1 simulation: a layered slab with one spherical inclusion.
+
+        layers: rows (n, mua, mus, flc, AbsCExi, AbsCEmi, eY, g, d) as in the .mci layer lines;
+        inclusion: (x, y, z, r, n, mua, mus, flc, AbsCExi, AbsCEmi, eY, g); derived optics,
+        layer bounds and the fhd grid computed like the parser (CUDAMCFLio.cu:427-526).
+        """
+        obj = cls()
+        s = obj.s
+        f32 = np.float32
+        rows = np.asarray(layers, dtype=np.float32).reshape(-1, 9)
+        nl = len(rows)
+        Ls = (Layer * (nl + 2))()
+        Ls[0].n = f32(n_above)
+        Ls[nl + 1].n = f32(n_below)
+        dtot = f32(0)
+        for i, r in enumerate(rows):
+            n, mua, mus, flc, exi, emi, ey, g, d = [f32(v) for v in r]
+            L = Ls[i + 1]
+            L.n = n
+            L.mua = f32(mua + f32(flc * exi))
+            L.muaf = f32(mua + f32(flc * emi))
+            L.eY, L.flc, L.g = ey, flc, g
+            L.z_min = dtot
+            dtot = f32(dtot + d)
+            L.z_max = dtot
+            L.mutr = np.finfo(f32).max if mus == 0 else f32(f32(1) / f32(mua + mus))
+            L.albedof = f32(mus / f32(f32(flc * exi) + mus))
+        x, y, z, r, n, mua, mus, flc, exi, emi, ey, g = [f32(v) for v in inclusion]
+        inc = s.inclusion
+        inc.x, inc.y, inc.z, inc.r, inc.n = x, y, z, r, n
+        inc.mua = f32(mua + f32(flc * exi))
+        inc.muaf = f32(mua + f32(flc * emi))
+        inc.flc, inc.eY, inc.g = flc, ey, g
+        inc.mutr = np.finfo(f32).max if mus == 0 else f32(f32(1) / f32(mua + mus))
+        with np.errstate(invalid="ignore", divide="ignore"):  # 0/0 as the parser computes it
+            inc.albedof = f32(mus / f32(f32(flc * exi) + mus))
+        B = (Bulk * 1)()
+        lab = np.zeros(1, np.uint8)
+        obj._keep += [Ls, B, lab]
+        s.number_of_photons = photons
+        s.n_layers = nl
+        s.layers = C.cast(Ls, C.POINTER(Layer))
+        s.n_bulks = 0
+        s.bulks = C.cast(B, C.POINTER(Bulk))
+        s.bulk_info = lab.ctypes.data_as(C.POINTER(C.c_uint8))
+        s.bulk_method = 1
+        s.grid_size = int(grid_size)
+        s.esp = dtot
+        s.nx = s.ny = int(f32(f32(4) * dtot) * f32(grid_size))
+        s.nz = int(f32(dtot * f32(grid_size)))
+        s.halfx = s.halfy = f32(2) * dtot
+        s.det.dx, s.det.dy, s.det.nx, s.det.ny, s.det.x0, s.det.y0 = det
+        s.xi, s.yi, s.zi = source
+        s.dir = 1.0 if src_kind != SOURCE_ISOTROPIC else 0.0
+        s.src_kind, s.src_r = src_kind, src_r
+        s.fhd_activated = 1 if fhd else 0
+        rr = (float(n_above) - float(rows[0][0])) / (float(n_above) + float(rows[0][0]))
+        s.start_weight = int(float(0xFFFFFFFF) * (1 - rr * rr))
+        s.outp_filename = outp.encode()
+        if temporal is not None:
+            (x0, y0, ndx, ndy, sx, sy, rad, rort, bins, tmax) = temporal
+            s.do_temp_sim = 1
+            d = s.det
+            d.x0_temp_det, d.y0_temp_det, d.x_temp_numdets, d.y_temp_numdets = x0, y0, ndx, ndy
+            d.x_temp_sepdets, d.y_temp_sepdets, d.temp_det_r, d.temp_rort = sx, sy, rad, rort
+            d.temp_bins, d.max_temp = bins, tmax
+        return obj
+
     @property
     def nvox(self):
         return self.s.nx * self.s.ny * self.s.nz

@@ -61,8 +61,14 @@ struct MediumDev {
   const uint8_t* labels; /* nx*ny*nz, x fastest */
   int32_t nx, ny, nz;
   uint32_t nxy;  /* nx*ny */
-  uint32_t last; /* n_bulks + 1 */
+  uint32_t last; /* n_bulks + 1 (voxel medium), n_layers + 1 (layered) */
   float esp, gs, halfx, halfy;
+  /* bulk method 1 (layered slab, n_layers > 0): materials 0..n_layers+1 are
+     the layers, material n_layers+2 the spherical inclusion; zb holds z_min,
+     z_max of each layer */
+  const float* zb;
+  uint32_t n_layers;
+  float inc_x, inc_y, inc_z, inc_r2;
 };
 
 struct DetDev {
@@ -89,6 +95,7 @@ struct JobDev {
   float xi, yi, zi, src_r;
   uint32_t start_weight;
   uint32_t mat_base; /* first entry of this job's material table in the launch table */
+  uint32_t start_lab; /* layered medium: the layer holding zi */
   uint32_t flags;
   double exit_scale;
   unsigned long long* Rd;
@@ -141,9 +148,9 @@ struct LaunchArgs {
   uint32_t refill_min; /* a wave refills when this many of its lanes are idle (1..64) */
 };
 
-/* Dynamic LDS of a launch: per-job totals, fhd tile, material table. */
-static inline uint32_t mcfl_lds_bytes(uint32_t nslots, uint32_t tile_n, uint32_t n_mats) {
-  return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * 32u;
+/* Dynamic LDS of a launch: per-job totals, fhd tile, material table, layer bounds. */
+static inline uint32_t mcfl_lds_bytes(uint32_t nslots, uint32_t tile_n, uint32_t n_mats, uint32_t n_zb) {
+  return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * 32u + n_zb * 8u;
 }
 
 #endif

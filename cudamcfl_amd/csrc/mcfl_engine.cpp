@@ -45,6 +45,7 @@ struct Medium {
 
 mcfl_status medium_make(const mcfl_sim* s, Medium* m) {
   MCFL_STCHK(validate(s));
+  if (s->bulk_method != 2) return MCFL_ERR_UNSUPPORTED;  // batches and serving take voxel media
   m->sim = *s;
   m->nvox = nvox_of(s);
   m->npix = npix_of(s);

@@ -46,24 +46,54 @@ void release(DevBuf& b) {
 
 // Per-material constants of the kernel (mcfl_device.h MatDev).  emission:
 // mua <- muaf (CUDAMCFLmain.cu:458-461).
+static void mat_of(float mutr, float mua, float g, float n, MatDev& m) {
+  const float g2 = g * g;
+  m.mutr = mutr;
+  m.mua_mutr = mua * mutr;
+  m.n = n;
+  m.onemg = 1.0f - g;
+  m.onemg2 = 1.0f - g2;
+  m.onepg2 = 1.0f + g2;
+  m.twog = 2.0f * g;
+  m.inv2g = (g != 0.0f) ? 1.0f / (2.0f * g) : 0.0f;
+}
+
+// Layered medium (bulk method 1): layers 0..n_layers+1, then the inclusion.
 void build_mats(const mcfl_sim* s, bool emission, std::vector<MatDev>& out) {
+  if (s->bulk_method == 1) {
+    const uint32_t n = s->n_layers + 2;
+    out.resize(n + 1);
+    for (uint32_t i = 0; i < n; ++i) {
+      const mcfl_layer& l = s->layers[i];
+      mat_of(l.mutr, emission ? l.muaf : l.mua, l.g, l.n, out[i]);
+    }
+    const mcfl_inclusion& c = s->inclusion;
+    mat_of(c.mutr, emission ? c.muaf : c.mua, c.g, c.n, out[n]);
+    return;
+  }
   const uint32_t n = s->n_bulks + 2;
   out.resize(n);
   for (uint32_t i = 0; i < n; ++i) {
     const mcfl_bulk& b = s->bulks[i];
-    const float mua = emission ? b.muaf : b.mua;
-    const float g = b.g;
-    const float g2 = g * g;
-    MatDev& m = out[i];
-    m.mutr = b.mutr;
-    m.mua_mutr = mua * b.mutr;
-    m.n = b.n;
-    m.onemg = 1.0f - g;
-    m.onemg2 = 1.0f - g2;
-    m.onepg2 = 1.0f + g2;
-    m.twog = 2.0f * g;
-    m.inv2g = (g != 0.0f) ? 1.0f / (2.0f * g) : 0.0f;
+    mat_of(b.mutr, emission ? b.muaf : b.mua, b.g, b.n, out[i]);
   }
+}
+
+// Layer bounds (z_min, z_max) of a layered medium, 0 above and n_layers+1 below.
+void layer_bounds(const mcfl_sim* s, std::vector<float>& zb) {
+  zb.assign(2 * ((size_t)s->n_layers + 2), 0.0f);
+  for (uint32_t i = 0; i < s->n_layers + 2; ++i) {
+    zb[2 * i] = s->layers[i].z_min;
+    zb[2 * i + 1] = s->layers[i].z_max;
+  }
+}
+
+// The layer LaunchPhoton starts a photon in (CUDAMCFLtransport.cu:547-560),
+// 0 when the source depth is in none of the layers 1..n_layers.
+uint32_t start_layer(const mcfl_sim* s) {
+  for (uint32_t l = 1; l <= s->n_layers; ++l)
+    if (s->zi < s->layers[l].z_max && s->zi >= s->layers[l].z_min) return l;
+  return 0;
 }
 
 size_t nvox_of(const mcfl_sim* s) { return (size_t)s->nx * (size_t)s->ny * (size_t)s->nz; }
@@ -74,28 +104,45 @@ size_t ntime_of(const mcfl_sim* s) {
 
 mcfl_status validate(const mcfl_sim* s) {
   if (!s) return MCFL_ERR_ARG;
-  if (s->bulk_method != 2) return MCFL_ERR_UNSUPPORTED;
-  if (s->nx <= 0 || s->ny <= 0 || s->nz <= 0 || !s->bulk_info || !s->bulks) return MCFL_ERR_ARG;
+  if (s->bulk_method != 1 && s->bulk_method != 2) return MCFL_ERR_UNSUPPORTED;
+  if (s->nx <= 0 || s->ny <= 0 || s->nz <= 0) return MCFL_ERR_ARG;
   if (s->grid_size <= 0 || !(s->esp > 0.0f)) return MCFL_ERR_ARG;
-  if (s->n_bulks < 1 || s->n_bulks > 98) return MCFL_ERR_CAPACITY;
   if (s->det.nx <= 0 || s->det.ny <= 0) return MCFL_ERR_ARG;
   if (s->do_temp_sim && (s->det.temp_bins == 0 || s->det.max_temp == 0)) return MCFL_ERR_ARG;
   const size_t nv = nvox_of(s);
   if (nv > 0xFFFFFFFFull) return MCFL_ERR_CAPACITY;
   if (s->nx >= (1 << 24) || s->ny >= (1 << 24)) return MCFL_ERR_CAPACITY;  // 24-bit index products in the kernel
+  if (s->bulk_method == 1) {
+    if (!s->layers) return MCFL_ERR_ARG;
+    if (s->n_layers < 1 || s->n_layers > 98) return MCFL_ERR_CAPACITY;
+    if (start_layer(s) == 0) return MCFL_ERR_ARG;  // the reference leaves the layer undefined
+    return MCFL_OK;
+  }
+  if (!s->bulk_info || !s->bulks) return MCFL_ERR_ARG;
+  if (s->n_bulks < 1 || s->n_bulks > 98) return MCFL_ERR_CAPACITY;
   const uint8_t last = (uint8_t)(s->n_bulks + 1);
   for (size_t i = 0; i < nv; ++i)
     if (s->bulk_info[i] > last) return MCFL_ERR_LABEL;
   return MCFL_OK;
 }
 
-void fill_medium(const mcfl_sim* s, const uint8_t* d_labels, MediumDev* md) {
+void fill_medium(const mcfl_sim* s, const uint8_t* d_labels, MediumDev* md, const float* d_zb) {
+  memset(md, 0, sizeof *md);
   md->labels = d_labels;
   md->nx = s->nx;
   md->ny = s->ny;
   md->nz = s->nz;
   md->nxy = (uint32_t)s->nx * (uint32_t)s->ny;
   md->last = s->n_bulks + 1;
+  if (s->bulk_method == 1) {
+    md->last = s->n_layers + 1;
+    md->n_layers = s->n_layers;
+    md->zb = d_zb;
+    md->inc_x = s->inclusion.x;
+    md->inc_y = s->inclusion.y;
+    md->inc_z = s->inclusion.z;
+    md->inc_r2 = s->inclusion.r * s->inclusion.r;
+  }
   md->esp = s->esp;
   md->gs = (float)s->grid_size;
   md->halfx = s->halfx;
@@ -137,10 +184,14 @@ int src_kind_of(const mcfl_sim* s) {
   return s->src_kind == MCFL_SOURCE_DISK ? MCFL_SRC_DISK : MCFL_SRC_PENCIL;
 }
 
+// Layer-bound entries of a launch's dynamic LDS (layered media only).
+static uint32_t n_zb_of(const LaunchArgs& A) { return A.med.n_layers ? A.med.n_layers + 2u : 0u; }
+
 int grid_blocks(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob) {
   const uint32_t nslots = A.work == MCFL_WORK_JOBS ? A.njobs : 1u;
   int bpc = 0;
-  if (mcfl_transport_occupancy(A.work, temporal, perjob, mcfl_lds_bytes(nslots, A.tile_n, A.n_mats), &bpc) != 0 ||
+  if (mcfl_transport_occupancy(A.work, temporal, perjob, A.med.n_layers ? 1 : 0,
+                               mcfl_lds_bytes(nslots, A.tile_n, A.n_mats, n_zb_of(A)), &bpc) != 0 ||
       bpc <= 0)
     bpc = 1;
   return c->n_cu * bpc;
@@ -150,10 +201,11 @@ void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi,
                  int perjob) {
   A.tile_n = 0;
   const uint32_t nslots = A.work == MCFL_WORK_JOBS ? A.njobs : 1u;
-  const int bpc = c->bpc[A.work][temporal ? 1 : 0][perjob ? 1 : 0];
+  const int layered = A.med.n_layers ? 1 : 0;
+  const int bpc = c->bpc[A.work][temporal ? 1 : 0][perjob ? 1 : 0][layered];
   if (bpc <= 0) return;
   const size_t per_wg = c->lds_per_cu / (size_t)bpc;
-  const size_t other = mcfl_lds_bytes(nslots, 0, A.n_mats);
+  const size_t other = mcfl_lds_bytes(nslots, 0, A.n_mats, n_zb_of(A));
   if (per_wg <= other + 64 * 8) return;
   const uint32_t cap = (uint32_t)std::min<size_t>((per_wg - other) / 8, 1u << 15);
   // box 2r x 2r x r around the source (the photon cloud is about as deep as it is wide)
@@ -176,7 +228,8 @@ void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi,
   // keep the register-limited occupancy
   int got = 0;
   while (A.tile_n > 0 &&
-         (mcfl_transport_occupancy(A.work, temporal, perjob, mcfl_lds_bytes(nslots, A.tile_n, A.n_mats), &got) != 0 ||
+         (mcfl_transport_occupancy(A.work, temporal, perjob, layered, mcfl_lds_bytes(nslots, A.tile_n, A.n_mats, n_zb_of(A)),
+                                   &got) != 0 ||
           got < bpc)) {
     if (A.tnz <= 1) {
       A.tile_n = 0;
@@ -243,10 +296,18 @@ bool read_multipliers(const char* path, uint32_t n, std::vector<uint32_t>& out) 
 }
 
 mcfl_status medium_upload(mcfl_ctx* c, const mcfl_sim* s, MediumDev* md) {
+  if (s->bulk_method == 1) {  // layered: no labels, layer bounds instead
+    std::vector<float> zb;
+    layer_bounds(s, zb);
+    STCHK(ensure(c->zb, zb.size() * sizeof(float)));
+    HIPCHK(hipMemcpy(c->zb.p, zb.data(), zb.size() * sizeof(float), hipMemcpyHostToDevice));
+    fill_medium(s, nullptr, md, c->zb.as<const float>());
+    return MCFL_OK;
+  }
   const size_t nv = nvox_of(s);
   STCHK(ensure(c->labels, nv));
   HIPCHK(hipMemcpyAsync(c->labels.p, s->bulk_info, nv, hipMemcpyHostToDevice, c->stream));
-  fill_medium(s, c->labels.as<const uint8_t>(), md);
+  fill_medium(s, c->labels.as<const uint8_t>(), md, nullptr);
   return MCFL_OK;
 }
 
@@ -342,7 +403,10 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   for (int w = 0; w < 3; ++w)
     for (int t = 0; t < 2; ++t)
       for (int pj = 0; pj < 2; ++pj)
-        if (mcfl_transport_occupancy(w, t, pj, 0, &c->bpc[w][t][pj]) != 0) return fail(MCFL_ERR_DEVICE);
+        for (int ly = 0; ly < 2; ++ly) {
+          if (ly && (pj || w == MCFL_WORK_VOXELS)) continue;  // no such variant
+          if (mcfl_transport_occupancy(w, t, pj, ly, 0, &c->bpc[w][t][pj][ly]) != 0) return fail(MCFL_ERR_DEVICE);
+        }
   if (prop.maxSharedMemoryPerMultiProcessor > 0) c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(MCFL_ERR_DEVICE);
@@ -372,7 +436,7 @@ extern "C" void mcfl_ctx_destroy(mcfl_ctx* c) {
 extern "C" mcfl_status mcfl_ctx_info(mcfl_ctx* c, int* n_cu, int* bpc, char* name, int name_len) {
   if (!c) return MCFL_ERR_ARG;
   if (n_cu) *n_cu = c->n_cu;
-  if (bpc) *bpc = c->bpc[MCFL_WORK_JOBS][0][0];
+  if (bpc) *bpc = c->bpc[MCFL_WORK_JOBS][0][0][0];
   if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->name);
   return MCFL_OK;
 }
@@ -423,6 +487,7 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
   J.src_r = s->src_r;
   J.start_weight = s->start_weight;
   J.mat_base = 0;
+  J.start_lab = s->bulk_method == 1 ? start_layer(s) : 0u;
   J.Rd = static_cast<unsigned long long*>(c->Rd.p);
   J.Tt = static_cast<unsigned long long*>(c->Tt.p);
   J.tot = static_cast<unsigned long long*>(c->tot.p);
@@ -550,6 +615,11 @@ extern "C" mcfl_status mcfl_fl_plan(const mcfl_sim* s, const double* Fx, uint64_
                                     uint64_t* n_vox, double* unit) {
   if (!s || !Fx || !n_vox || !unit || nper == 0) return MCFL_ERR_ARG;
   STCHK(validate(s));
+  if (s->bulk_method == 1) {  // method 1 emits only where albedof < 0, i.e. nowhere (CUDAMCFLmain.cu:520,535)
+    *n_vox = 0;
+    *unit = 1.0;
+    return MCFL_OK;
+  }
   const size_t nv = nvox_of(s);
   const uint64_t cap = *n_vox;
   uint64_t n = 0;
@@ -580,6 +650,20 @@ extern "C" mcfl_status mcfl_run_fluorescence(mcfl_ctx* c, const mcfl_sim* s, con
   STCHK(validate(s));
   if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
   const size_t nv = nvox_of(s), npix = npix_of(s);
+  if (s->bulk_method == 1) {  // no emitting voxel (see mcfl_fl_plan): zero images
+    const size_t xy = (size_t)s->det.nx * (size_t)s->det.ny;
+    if (FlR) memset(FlR, 0, xy * sizeof(double));
+    if (FlT) memset(FlT, 0, xy * sizeof(double));
+    if (res) {
+      if (res->FlR_raw) memset(res->FlR_raw, 0, npix * 8);
+      if (res->FlT_raw) memset(res->FlT_raw, 0, npix * 8);
+      res->unit = 1.0;
+      res->voxels = res->photons = 0;
+      res->kernel_ms = 0.0;
+      memset(res->totals, 0, sizeof res->totals);
+    }
+    return MCFL_OK;
+  }
   std::vector<uint32_t> vox(nv);
   std::vector<double> scale(nv);
   uint64_t nvx = nv;

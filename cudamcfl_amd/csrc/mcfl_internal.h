@@ -13,7 +13,8 @@
 #include "../../include/mcfl.h"
 #include "mcfl_device.h"
 
-extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, uint32_t lds_bytes, int* blocks_per_cu);
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int layered, uint32_t lds_bytes,
+                                        int* blocks_per_cu);
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream);
 extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
 extern "C" int mcfl_reduce_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, hipStream_t stream);
@@ -39,7 +40,9 @@ size_t nvox_of(const mcfl_sim* s);
 size_t npix_of(const mcfl_sim* s);
 size_t ntime_of(const mcfl_sim* s);
 mcfl_status validate(const mcfl_sim* s);
-void fill_medium(const mcfl_sim* s, const uint8_t* d_labels, MediumDev* md);
+void fill_medium(const mcfl_sim* s, const uint8_t* d_labels, MediumDev* md, const float* d_zb = nullptr);
+void layer_bounds(const mcfl_sim* s, std::vector<float>& zb);
+uint32_t start_layer(const mcfl_sim* s);
 void fill_det(const mcfl_sim* s, DetDev* d, const float* d_tpos);
 void tdet_positions(const mcfl_sim* s, std::vector<float>& pos);
 int src_kind_of(const mcfl_sim* s);
@@ -51,7 +54,7 @@ struct mcfl_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
-  int bpc[3][2][2] = {};  // resident workgroups per CU [work][temporal][perjob], register-limited
+  int bpc[3][2][2][2] = {};  // resident workgroups per CU [work][temporal][perjob][layered], register-limited
   size_t lds_per_cu = 160 * 1024;
   uint32_t copies = MCFL_TALLY_COPIES;  // copies of the per-step volume tallies (1 or MCFL_TALLY_COPIES)
   int tile_side = 0;                    // lateral size of the fhd LDS tile, 0: from the LDS budget (env MCFL_TILE_SIDE)
@@ -59,7 +62,7 @@ struct mcfl_ctx {
   char name[256] = {};
   uint32_t* d_atab = nullptr;
   std::vector<uint32_t> atab;
-  mcfl_detail::DevBuf labels, mats, jobs, counter, tot, Rd, Tt, fhd, axyz, time, Rd2, Tt2, tpos, xs, as, vox, vscale;
+  mcfl_detail::DevBuf labels, zb, mats, jobs, counter, tot, Rd, Tt, fhd, axyz, time, Rd2, Tt2, tpos, xs, as, vox, vscale;
 };
 
 namespace mcfl_detail {

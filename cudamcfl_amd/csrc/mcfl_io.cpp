@@ -311,6 +311,12 @@ extern "C" int mcfl_read_simulation_data(const char* filename, mcfl_sim** simula
       I->albedof = ft[6] / (ft[7] * ft[8] + ft[6]);
       n1 = s->layers[0].n;
       n2 = s->layers[1].n;
+      // photon hitting density grid, as for method 2 (CUDAMCFLmem.cu:21-23)
+      s->nx = (int)(4 * s->esp * s->grid_size);
+      s->ny = (int)(4 * s->esp * s->grid_size);
+      s->nz = (int)(s->esp * s->grid_size);
+      s->halfx = 2 * s->esp;
+      s->halfy = 2 * s->esp;
       s->n_bulks = 0;
       s->bulks = static_cast<mcfl_bulk*>(calloc(1, sizeof(mcfl_bulk)));
       s->bulk_info = static_cast<uint8_t*>(calloc(1, 1));

@@ -79,6 +79,12 @@ __device__ __forceinline__ uint32_t cell_index(const MediumDev& m, Cell c) {
   return c.ix + __umul24(c.iy, (uint32_t)m.nx) + c.iz * m.nxy;
 }
 
+/* Inside the layered medium's spherical inclusion (CUDAMCFLtransport.cu:83-86, 201-204). */
+__device__ __forceinline__ bool in_sphere(const MediumDev& m, float x, float y, float z) {
+  const float dx = x - m.inc_x, dy = y - m.inc_y, dz = z - m.inc_z;
+  return (dx * dx + dy * dy) + dz * dz <= m.inc_r2;
+}
+
 typedef __attribute__((address_space(1))) unsigned long long gu64;
 
 __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v) {
@@ -91,7 +97,7 @@ __device__ __forceinline__ void lds_add_u64(unsigned long long* p, unsigned long
 
 /* LaunchPhoton, CUDAMCFLtransport.cu:460-562 (method 2). */
 __device__ void launch(const MediumDev& m, int kind, float xi, float yi, float zi, float src_r, uint32_t sw, Ph& p,
-                       uint64_t& x, uint32_t a) {
+                       uint64_t& x, uint32_t a, bool layered, uint32_t start_layer) {
   if (kind == MCFL_SRC_ISOTROPIC) {
     const float step = 0.999999f / m.gs;
     p.x = xi + step * mcfl_rand_oc(&x, a);
@@ -123,7 +129,9 @@ __device__ void launch(const MediumDev& m, int kind, float xi, float yi, float z
     p.dz = 1.0f;
     p.w = sw;
   }
-  if (p.z < m.esp)
+  if (layered) /* the layer holding the source depth (:547-560) */
+    p.lab = start_layer;
+  else if (p.z < m.esp)
     p.lab = (mcfl_fabs(p.x) < m.halfx && mcfl_fabs(p.y) < m.halfy) ? (uint32_t)m.labels[vox_of(m, p.x, p.y, p.z)] : 1u;
   else
     p.lab = m.last;
@@ -310,9 +318,9 @@ __device__ __forceinline__ uint32_t find_job(const LaunchArgs& A, uint64_t g, ui
 
 } /* namespace */
 
-template <int WORK, bool TEMPORAL, bool PERJOB>
+template <int WORK, bool TEMPORAL, bool PERJOB, bool LAYERED>
 __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WAVES_PER_EU))) void mcfl_transport_kernel(LaunchArgs A) {
-  /* dynamic LDS: [per-job totals | fhd tile | materials] */
+  /* dynamic LDS: [per-job totals | fhd tile | materials | layer bounds] */
   extern __shared__ __align__(16) unsigned long long lds[];
 
   const uint32_t tid = threadIdx.x;
@@ -320,7 +328,10 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
   unsigned long long* stot = lds;
   unsigned long long* stile = lds + nslots * MCFL_NTOT;
   MatDev* smat = reinterpret_cast<MatDev*>(stile + A.tile_n);
+  float* szb = reinterpret_cast<float*>(smat + A.n_mats);
   for (uint32_t i = tid; i < A.n_mats; i += MCFL_WG) smat[i] = A.mats[i];
+  if (LAYERED)
+    for (uint32_t i = tid; i < 2u * (A.med.n_layers + 2u); i += MCFL_WG) szb[i] = A.med.zb[i];
   for (uint32_t i = tid; i < nslots * MCFL_NTOT + A.tile_n; i += MCFL_WG) lds[i] = 0ull;
   __syncthreads();
 
@@ -418,9 +429,9 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
             const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
             const uint32_t iz = v / nxy, rem = v - iz * nxy, iy = rem / (uint32_t)m.nx, ix = rem - iy * (uint32_t)m.nx;
             launch(m, MCFL_SRC_ISOTROPIC, ((float)ix / m.gs) - m.halfx, ((float)iy / m.gs) - m.halfy,
-                   ((float)iz / m.gs), 0.0f, 0u, p, x, a);
+                   ((float)iz / m.gs), 0.0f, 0u, p, x, a, false, 0u);
           } else {
-            launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a);
+            launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a, LAYERED, J.start_lab);
           }
         } else if (WORK == MCFL_WORK_VOXELS) {
           const uint64_t j = g / A.per_job;
@@ -434,7 +445,7 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
           const float xi = ((float)ix / m.gs) - m.halfx;
           const float yi = ((float)iy / m.gs) - m.halfy;
           const float zi = ((float)iz / m.gs);
-          launch(m, MCFL_SRC_ISOTROPIC, xi, yi, zi, 0.0f, 0u, p, x, a);
+          launch(m, MCFL_SRC_ISOTROPIC, xi, yi, zi, 0.0f, 0u, p, x, a, false, 0u);
         } else { /* reference lane streams */
           const JobDev& J = A.jobs[0];
           lane_item = (uint32_t)g;
@@ -446,7 +457,7 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
           fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
           axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
           uint64_t xl = x; /* LaunchPhoton_Global does not store the advanced state */
-          launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, xl, a);
+          launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, xl, a, LAYERED, J.start_lab);
         }
         unsigned long long* st = stot + job * MCFL_NTOT;
         lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
@@ -459,39 +470,80 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
     if (__ballot(alive) == 0ull) break;
     if (!alive) continue;
 
-    /* ---- one transport step, CUDAMCFLtransport.cu:289-435 ---- */
+    /* ---- one transport step: MCd3D CUDAMCFLtransport.cu:289-435 (voxel
+       medium) or MCd :79-226 (layered slab with one spherical inclusion) ---- */
     const MatDev* mats = smat + mbase;
     unsigned long long* st = stot + job * MCFL_NTOT;
-    const MatDev mc = mats[p.lab];
     float s;
     uint32_t nb = p.lab;
+    MatDev mm; /* material of the absorption and Spin */
     ++iters;
-    if (mc.mutr != kFltMax) {
-      s = -mcfl_log(mcfl_rand_oc(&x, a)) * mc.mutr;
-      if (mcfl_fma(s, p.dz, p.z) < 0.0f) {
-        nb = 0;
-        s = -p.z / p.dz;
+    bool in;
+    Cell cl;
+    uint32_t vi;
+    if constexpr (LAYERED) {
+      const uint32_t L = p.lab;
+      const float zlo = szb[2u * L], zhi = szb[2u * L + 1u];
+      const MatDev ml = mats[L];
+      if (in_sphere(m, p.x, p.y, p.z)) { /* :83-92 */
+        const float mutr = mats[m.n_layers + 2u].mutr;
+        s = (mutr != kFltMax) ? -mcfl_log(mcfl_rand_oc(&x, a)) * mutr : 2.0f;
+      } else { /* :94-100: deterministic step outside the inclusion */
+        s = (ml.mutr != kFltMax) ? ml.mutr : 2.0f;
       }
-      if (mcfl_fma(s, p.dz, p.z) > m.esp) {
-        nb = m.last;
-        s = (m.esp - p.z) / p.dz;
+      int nl = (int)L;
+      if (mcfl_fma(s, p.dz, p.z) < zlo) { /* :106-115 */
+        --nl;
+        s = (zlo - p.z) / p.dz;
+      }
+      if (mcfl_fma(s, p.dz, p.z) >= zhi) {
+        ++nl;
+        s = (zhi - p.z) / p.dz;
       }
       p.x = mcfl_fma(p.dx, s, p.x);
       p.y = mcfl_fma(p.dy, s, p.y);
       p.z = mcfl_fma(p.dz, s, p.z);
-      if (TEMPORAL) p.tof += mcfl_d2u64_rz((double)(s * mc.n) / kCcmfs);
+      if (TEMPORAL) p.tof += mcfl_d2u64_rz((double)(s * ml.n) / kCcmfs);
+      if (p.z > zhi) p.z = zhi; /* :125-126 */
+      if (p.z < zlo) p.z = zlo;
+      in = in_grid(m, p.x, p.y, p.z);
+      cl = cell_in(m, p.x, p.y, p.z);
+      vi = in ? cell_index(m, cl) : 0u;
+      nb = (uint32_t)nl;
+      if (nb != L) { /* :141-145: no absorption or scattering after a layer change */
+        s = 0.0f;
+        reflect(mats, p, nb, x, a);
+      }
+      mm = in_sphere(m, p.x, p.y, p.z) ? mats[m.n_layers + 2u] : mats[p.lab]; /* :200-224 */
     } else {
-      s = 100.0f;
-      nb = glass_march<TEMPORAL>(m, p, p.lab, s);
-    }
-    const bool in = in_grid(m, p.x, p.y, p.z);
-    const Cell cl = cell_in(m, p.x, p.y, p.z);
-    const uint32_t vi = in ? cell_index(m, cl) : 0u;
-    if (nb == p.lab && mc.mutr != kFltMax) nb = in ? (uint32_t)m.labels[vi] : 1u; /* :319-333 */
-    MatDev mm = mc; /* material after a possible interface crossing */
-    if (nb != p.lab) {
-      reflect(mats, p, nb, x, a);
-      mm = mats[p.lab];
+      const MatDev mc = mats[p.lab];
+      if (mc.mutr != kFltMax) {
+        s = -mcfl_log(mcfl_rand_oc(&x, a)) * mc.mutr;
+        if (mcfl_fma(s, p.dz, p.z) < 0.0f) {
+          nb = 0;
+          s = -p.z / p.dz;
+        }
+        if (mcfl_fma(s, p.dz, p.z) > m.esp) {
+          nb = m.last;
+          s = (m.esp - p.z) / p.dz;
+        }
+        p.x = mcfl_fma(p.dx, s, p.x);
+        p.y = mcfl_fma(p.dy, s, p.y);
+        p.z = mcfl_fma(p.dz, s, p.z);
+        if (TEMPORAL) p.tof += mcfl_d2u64_rz((double)(s * mc.n) / kCcmfs);
+      } else {
+        s = 100.0f;
+        nb = glass_march<TEMPORAL>(m, p, p.lab, s);
+      }
+      in = in_grid(m, p.x, p.y, p.z);
+      cl = cell_in(m, p.x, p.y, p.z);
+      vi = in ? cell_index(m, cl) : 0u;
+      if (nb == p.lab && mc.mutr != kFltMax) nb = in ? (uint32_t)m.labels[vi] : 1u; /* :319-333 */
+      mm = mc; /* material after a possible interface crossing */
+      if (nb != p.lab) {
+        reflect(mats, p, nb, x, a);
+        mm = mats[p.lab];
+      }
     }
 
     if ((flags & MCFL_F_FHD) && in) {
@@ -520,8 +572,10 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
       s = 0.0f;
     }
     if (s > 0.0f) {
-      const uint32_t w = mcfl_f2u_ru(mm.mua_mutr * (float)p.w);
-      if (w > 0u) {
+      /* voxel medium: w = ceil(mua mutr weight), killed at w = 0 (:425-431);
+         layered: rounded to nearest, no kill (:205-223) */
+      const uint32_t w = LAYERED ? mcfl_f2u_rn(mm.mua_mutr * (float)p.w) : mcfl_f2u_ru(mm.mua_mutr * (float)p.w);
+      if (LAYERED || w > 0u) {
         p.w -= w;
         accA += w;
         if ((flags & MCFL_F_AXYZ) && in) {
@@ -579,7 +633,7 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
           --lane_left;
           accA = 0;
           iters = 0;
-          launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a);
+          launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a, LAYERED, J.start_lab);
           lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
           lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
           alive = true;
@@ -628,33 +682,49 @@ __global__ void mcfl_reduce_copies_kernel(unsigned long long* p, uint64_t n, uin
 /* ------------------------------------------------------------------ host */
 
 namespace {
-template <int W, bool T, bool P>
+template <int W, bool T, bool P, bool L>
 const void* kfn() {
-  return (const void*)mcfl_transport_kernel<W, T, P>;
+  return (const void*)mcfl_transport_kernel<W, T, P, L>;
 }
-const void* kernel_of(int work, int temporal, int perjob) {
-  if (work == MCFL_WORK_JOBS) {
-    if (perjob) return temporal ? kfn<MCFL_WORK_JOBS, true, true>() : kfn<MCFL_WORK_JOBS, false, true>();
-    return temporal ? kfn<MCFL_WORK_JOBS, true, false>() : kfn<MCFL_WORK_JOBS, false, false>();
+/* perjob selects mixed-media launches (jobs only); layered the bulk-method-1
+   medium (jobs and reference lanes, one medium per launch) */
+const void* kernel_of(int work, int temporal, int perjob, int layered) {
+  if (layered) {
+    if (work == MCFL_WORK_JOBS)
+      return temporal ? kfn<MCFL_WORK_JOBS, true, false, true>() : kfn<MCFL_WORK_JOBS, false, false, true>();
+    if (work == MCFL_WORK_LANES)
+      return temporal ? kfn<MCFL_WORK_LANES, true, false, true>() : kfn<MCFL_WORK_LANES, false, false, true>();
+    return nullptr;
   }
-  if (work == MCFL_WORK_VOXELS) return temporal ? kfn<MCFL_WORK_VOXELS, true, false>() : kfn<MCFL_WORK_VOXELS, false, false>();
-  return temporal ? kfn<MCFL_WORK_LANES, true, false>() : kfn<MCFL_WORK_LANES, false, false>();
+  if (work == MCFL_WORK_JOBS) {
+    if (perjob)
+      return temporal ? kfn<MCFL_WORK_JOBS, true, true, false>() : kfn<MCFL_WORK_JOBS, false, true, false>();
+    return temporal ? kfn<MCFL_WORK_JOBS, true, false, false>() : kfn<MCFL_WORK_JOBS, false, false, false>();
+  }
+  if (work == MCFL_WORK_VOXELS)
+    return temporal ? kfn<MCFL_WORK_VOXELS, true, false, false>() : kfn<MCFL_WORK_VOXELS, false, false, false>();
+  return temporal ? kfn<MCFL_WORK_LANES, true, false, false>() : kfn<MCFL_WORK_LANES, false, false, false>();
 }
 }  // namespace
 
-extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, uint32_t lds_bytes, int* blocks_per_cu) {
-  hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel_of(work, temporal, perjob), MCFL_WG, lds_bytes);
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int layered, uint32_t lds_bytes,
+                                        int* blocks_per_cu) {
+  const void* k = kernel_of(work, temporal, perjob, layered);
+  if (!k) return -1;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, MCFL_WG, lds_bytes);
   return e == hipSuccess ? 0 : (int)e;
 }
 
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream) {
   if (A->work < 0 || A->work > 2) return -1;
+  const int layered = A->med.n_layers > 0u ? 1 : 0;
+  const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered);
+  if (!k || (layered && perjob)) return -1;
   void* args[] = {const_cast<LaunchArgs*>(A)};
   const uint32_t nslots = A->work == MCFL_WORK_JOBS ? A->njobs : 1u;
-  hipError_t e = hipLaunchKernel(kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0),
-                                 dim3((unsigned)nblocks), dim3(MCFL_WG), args,
-                                 mcfl_lds_bytes(nslots, A->tile_n, A->n_mats), stream);
+  hipError_t e = hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(MCFL_WG), args,
+                                 mcfl_lds_bytes(nslots, A->tile_n, A->n_mats, layered ? A->med.n_layers + 2u : 0u),
+                                 stream);
   return e == hipSuccess ? 0 : (int)e;
 }
 

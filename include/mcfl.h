@@ -36,7 +36,7 @@ typedef enum mcfl_status {
   MCFL_ERR_LABEL = 6,       /* voxel label outside 0..n_bulks+1 */
   MCFL_ERR_NOMEM = 7,       /* host or device allocation failed */
   MCFL_ERR_DEVICE = 8,      /* HIP runtime error / no device */
-  MCFL_ERR_UNSUPPORTED = 9, /* feature outside this build (e.g. bulk method 1 on GPU) */
+  MCFL_ERR_UNSUPPORTED = 9, /* feature outside this build (e.g. a layered medium in a batch) */
   MCFL_ERR_CAPACITY = 10    /* request exceeds a stated limit */
 } mcfl_status;
 
@@ -93,7 +93,7 @@ typedef struct mcfl_sim { /* SimulationStruct, CUDAMCFL.h:129-170 */
   float n_down;
   int32_t fhd_activated;
   int32_t do_fl_sim;
-  int32_t bulk_method;
+  int32_t bulk_method; /* 1: layered slab + one sphere (MCd), 2: voxel labels (MCd3D) */
   uint8_t* bulk_info; /* nx*ny*nz labels, x fastest (the reference stores int16) */
   char bulkinfo_filename[MCFL_STR_LEN];
   int32_t do_temp_sim;

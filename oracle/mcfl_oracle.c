@@ -31,29 +31,31 @@ typedef struct {
 typedef struct {
   const orc_sim* s;
   mat_t mt[256];
+  mat_t inc; /* method 1: the spherical inclusion */
   uint32_t last;
   size_t nvox, npix;
 } ctx_t;
+
+static void mat_init(mat_t* m, const orc_bulk* b) {
+  float g = b->g;
+  float g2 = g * g;
+  m->mutr = b->mutr;
+  m->mua_mutr = b->mua * b->mutr;
+  m->n = b->n;
+  m->g = g;
+  m->onemg = 1.0f - g;
+  m->onemg2 = 1.0f - g2;
+  m->onepg2 = 1.0f + g2;
+  m->twog = 2.0f * g;
+  m->inv2g = (g != 0.0f) ? 1.0f / (2.0f * g) : 0.0f;
+}
 
 static void ctx_init(ctx_t* c, const orc_sim* s) {
   memset(c, 0, sizeof *c);
   c->s = s;
   c->last = s->n_bulks + 1u;
-  for (uint32_t i = 0; i < s->n_bulks + 2u && i < 256u; ++i) {
-    const orc_bulk* b = &s->bulks[i];
-    mat_t* m = &c->mt[i];
-    float g = b->g;
-    float g2 = g * g;
-    m->mutr = b->mutr;
-    m->mua_mutr = b->mua * b->mutr;
-    m->n = b->n;
-    m->g = g;
-    m->onemg = 1.0f - g;
-    m->onemg2 = 1.0f - g2;
-    m->onepg2 = 1.0f + g2;
-    m->twog = 2.0f * g;
-    m->inv2g = (g != 0.0f) ? 1.0f / (2.0f * g) : 0.0f;
-  }
+  for (uint32_t i = 0; i < s->n_bulks + 2u && i < 256u; ++i) mat_init(&c->mt[i], &s->bulks[i]);
+  if (s->layered) mat_init(&c->inc, &s->inc);
   c->nvox = (size_t)s->nx * (size_t)s->ny * (size_t)s->nz;
   c->npix = (size_t)s->det_nx * (size_t)(s->det_ny + 1);
 }
@@ -156,7 +158,8 @@ static void orc_launch(const ctx_t* c, int kind, float xi, float yi, float zi, o
     p->dz = 1.0f;
     p->weight = s->start_weight; /* :527 */
   }
-  p->bulkpos = launch_label(c, p->x, p->y, p->z);
+  /* method 2: label at the entry point; method 1: the layer holding zi (:547-560) */
+  p->bulkpos = s->layered ? s->start_layer : launch_label(c, p->x, p->y, p->z);
   p->tof = 0;
 }
 
@@ -363,10 +366,90 @@ static void exit_tally(const ctx_t* c, const orc_photon* p, int top, tl_t* t) {
   }
 }
 
+/* PhotonSurvive, CUDAMCFLtransport.cu:850-864.  Returns 0 when the photon dies. */
+static int survive(orc_photon* p, uint64_t* x, uint32_t a, tl_t* t) {
+  if (p->weight > WEIGHTI) return 1;
+  if (p->weight == 0u) return 0;
+  if (mcfl_rand_co(x, a) < CHANCE) {
+    uint32_t nw = mcfl_f2u_rn((float)p->weight / CHANCE);
+    t->tot[ORC_T_ROUL_GAIN] += (uint64_t)(nw - p->weight);
+    p->weight = nw;
+    return 1;
+  }
+  t->tot[ORC_T_ROUL_LOSS] += p->weight;
+  p->weight = 0u;
+  return 0;
+}
+
+/* Inside the method-1 inclusion (CUDAMCFLtransport.cu:83-86, 201-204). */
+static int in_sphere(const orc_sim* s, float x, float y, float z) {
+  float dx = x - s->inc_x, dy = y - s->inc_y, dz = z - s->inc_z;
+  return (dx * dx + dy * dy) + dz * dz <= s->inc_r * s->inc_r;
+}
+
+/* One iteration of the MCd loop body (bulk method 1, CUDAMCFLtransport.cu:79-226):
+ * step sampled only inside the inclusion, deterministic mutr outside (:99);
+ * layer bounds clip the step (:104-115); the step length is zeroed on any
+ * layer change, so a reflected or transmitted photon neither absorbs nor
+ * scatters at the interface (:142); absorption rounds to nearest with no kill
+ * at zero (:205-223). */
+static int orc_step_layered(const ctx_t* c, orc_photon* p, uint64_t* x, uint32_t a, tl_t* t) {
+  const orc_sim* s = c->s;
+  const uint32_t L = p->bulkpos;
+  const mat_t* ml = &c->mt[L];
+  float st;
+  t->tot[ORC_T_STEPS] += 1;
+  if (in_sphere(s, p->x, p->y, p->z))
+    st = (c->inc.mutr != FLT_MAX) ? -mcfl_log(mcfl_rand_oc(x, a)) * c->inc.mutr : 2.0f; /* :87-91 */
+  else
+    st = (ml->mutr != FLT_MAX) ? ml->mutr : 2.0f; /* :94-100 */
+  int nl = (int)L;
+  if (mcfl_fma(st, p->dz, p->z) < s->z_min[L]) { /* :106-109 */
+    nl--;
+    st = (s->z_min[L] - p->z) / p->dz;
+  }
+  if (mcfl_fma(st, p->dz, p->z) >= s->z_max[L]) { /* :112-115 */
+    nl++;
+    st = (s->z_max[L] - p->z) / p->dz;
+  }
+  p->x = mcfl_fma(p->dx, st, p->x); /* :118-120 */
+  p->y = mcfl_fma(p->dy, st, p->y);
+  p->z = mcfl_fma(p->dz, st, p->z);
+  p->tof += mcfl_d2u64_rz((double)(st * ml->n) / C_CMFS); /* :123 */
+  if (p->z > s->z_max[L]) p->z = s->z_max[L];             /* :125-126 */
+  if (p->z < s->z_min[L]) p->z = s->z_min[L];
+  int in = inside(s, p->x, p->y, p->z);
+  uint32_t vi = in ? vindex(s, p->x, p->y, p->z) : 0u;
+  if (s->fhd_on && in) t->fhd[vi] += p->weight; /* :129-138 */
+  if (nl != (int)L) {                           /* :141-196 */
+    st = 0.0f;
+    if (!orc_reflect(c, p, (uint32_t)nl, x, a)) {
+      if (nl == 0) { /* :147-170 diffuse reflectance */
+        exit_tally(c, p, 1, t);
+        p->weight = 0u;
+      }
+      if (nl > (int)s->n_bulks) { /* :172-194 diffuse transmittance */
+        exit_tally(c, p, 0, t);
+        p->weight = 0u;
+      }
+    }
+  }
+  if (st > 0.0f) { /* :200-224, the material at the end point */
+    const mat_t* m = in_sphere(s, p->x, p->y, p->z) ? &c->inc : &c->mt[p->bulkpos];
+    uint32_t w = mcfl_f2u_rn(m->mua_mutr * (float)p->weight);
+    p->weight -= w; /* the reference's underflow test on unsigned values is vacuous */
+    t->tot[ORC_T_A] += w;
+    if (s->axyz_on && in) t->axyz[vi] += w;
+    orc_spin(m, p, x, a);
+  }
+  return survive(p, x, a, t);
+}
+
 /* One iteration of the MCd3D loop body up to and including the roulette
  * (CUDAMCFLtransport.cu:289-435).  Returns 0 when the photon has died. */
 static int orc_step(const ctx_t* c, orc_photon* p, uint64_t* x, uint32_t a, tl_t* t) {
   const orc_sim* s = c->s;
+  if (s->layered) return orc_step_layered(c, p, x, a, t);
   const mat_t* m = &c->mt[p->bulkpos];
   float st;
   uint32_t nb;
@@ -425,18 +508,7 @@ static int orc_step(const ctx_t* c, orc_photon* p, uint64_t* x, uint32_t a, tl_t
     }
     orc_spin(mm, p, x, a);
   }
-  /* PhotonSurvive, CUDAMCFLtransport.cu:850-864 */
-  if (p->weight > WEIGHTI) return 1;
-  if (p->weight == 0u) return 0;
-  if (mcfl_rand_co(x, a) < CHANCE) {
-    uint32_t nw = mcfl_f2u_rn((float)p->weight / CHANCE);
-    t->tot[ORC_T_ROUL_GAIN] += (uint64_t)(nw - p->weight);
-    p->weight = nw;
-    return 1;
-  }
-  t->tot[ORC_T_ROUL_LOSS] += p->weight;
-  p->weight = 0u;
-  return 0;
+  return survive(p, x, a, t);
 }
 
 /* Run one photon from its launch to its death. */

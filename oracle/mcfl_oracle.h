@@ -17,6 +17,7 @@
  *   orc_move_to_boundary CUDAMCFLtransport.cu:702-847 (MoveToFirstBoundary)
  *   survive             CUDAMCFLtransport.cu:850-864 (PhotonSurvive)
  *   orc_step            CUDAMCFLtransport.cu:286-447 (one MCd3D iteration)
+ *   orc_step_layered    CUDAMCFLtransport.cu:79-226  (one MCd iteration, method 1)
  *   orc_run_lanes       CUDAMCFLmain.cu:60-165 + CUDAMCFLtransport.cu:565-582
  *                       (DoOneSimulation relaunch loop, per-lane streams)
  *   orc_fluor_reference CUDAMCFLmain.cu:409-594 (per-voxel emission loop)
@@ -65,6 +66,15 @@ typedef struct {
   uint32_t start_weight;
   /* tallies */
   int32_t fhd_on, axyz_on, sq_on;
+  /* bulk method 1 (layered slab with one spherical inclusion): bulks[] are
+     the layers 0..n_bulks+1 (n_bulks = number of layers, 0 above and
+     n_bulks+1 below), z_min/z_max their bounds, labels unused */
+  int32_t layered;
+  const float* z_min;
+  const float* z_max;
+  orc_bulk inc;
+  float inc_x, inc_y, inc_z, inc_r;
+  uint32_t start_layer; /* layer containing the source depth */
 } orc_sim;
 
 enum {
@@ -139,7 +149,8 @@ void orc_spin_test(float g, const float* dir_in, uint64_t* x, uint32_t a, float*
 void orc_backstep_test(const float* p, const float* d, const float* c, float r2, float* t1, float* t2);
 float orc_fresnel_test(float n1, float n2, float cos_i);
 
-/* fn: 0 log, 1 sin(2 pi t), 2 cos(2 pi t), 3 uint32 (bits of in) -> float toward zero, 4 1/sqrt */
+/* fn: 0 log, 1 sin(2 pi t), 2 cos(2 pi t), 3 uint32 (bits of in) -> float toward zero, 4 1/sqrt,
+   5 x*rsqrt(x), 6 renormalisation factor */
 void orc_math_eval(int fn, const float* in, float* out, size_t n);
 /* n draws of the [0,1) generator from state x: states after each step and the draws */
 void orc_mwc_draws(uint64_t x, uint32_t a, size_t n, uint64_t* states, float* co);

@@ -34,6 +34,9 @@ class Sim(C.Structure):
         ("src_kind", C.c_int32), ("xi", C.c_float), ("yi", C.c_float), ("zi", C.c_float), ("src_r", C.c_float),
         ("start_weight", C.c_uint32),
         ("fhd_on", C.c_int32), ("axyz_on", C.c_int32), ("sq_on", C.c_int32),
+        ("layered", C.c_int32), ("z_min", F32P), ("z_max", F32P), ("inc", Bulk),
+        ("inc_x", C.c_float), ("inc_y", C.c_float), ("inc_z", C.c_float), ("inc_r", C.c_float),
+        ("start_layer", C.c_uint32),
     ]
 
 
@@ -175,18 +178,35 @@ class FromProduct:
 
     def __init__(self, sim, emission=False, axyz=False, sq=False):
         s = sim.s
-        nb = s.n_bulks + 2
-        rows = []
-        for i in range(nb):
-            b = s.bulks[i]
-            mua = b.muaf if emission else b.mua
-            rows.append([b.mutr, mua, b.g, b.n, b.flc, b.muaf, b.eY, b.albedof])
-        self.bulks = (Bulk * nb)(*[Bulk(*map(float, r)) for r in np.asarray(rows, np.float32)])
-        self.labels = np.array(sim.labels, copy=True)
         o = Sim()
+        if s.bulk_method == 1:
+            # layered slab: layer optics as bulks, their bounds, the inclusion
+            nb = s.n_layers + 2
+            Ls = [s.layers[i] for i in range(nb)]
+            rows = [[L.mutr, L.mua, L.g, L.n, L.flc, L.muaf, L.eY, L.albedof] for L in Ls]
+            self.zmin = np.array([L.z_min for L in Ls], np.float32)
+            self.zmax = np.array([L.z_max for L in Ls], np.float32)
+            c = s.inclusion
+            o.layered = 1
+            o.z_min, o.z_max = ptr(self.zmin, F32P), ptr(self.zmax, F32P)
+            o.inc = Bulk(c.mutr, c.mua, c.g, c.n, c.flc, c.muaf, c.eY, c.albedof)
+            o.inc_x, o.inc_y, o.inc_z, o.inc_r = c.x, c.y, c.z, c.r
+            o.start_layer = next((l for l in range(1, s.n_layers + 1)
+                                  if self.zmin[l] <= np.float32(s.zi) < self.zmax[l]), 0)
+            self.labels = np.zeros(1, np.uint8)
+            o.n_bulks = s.n_layers
+        else:
+            nb = s.n_bulks + 2
+            rows = []
+            for i in range(nb):
+                b = s.bulks[i]
+                mua = b.muaf if emission else b.mua
+                rows.append([b.mutr, mua, b.g, b.n, b.flc, b.muaf, b.eY, b.albedof])
+            self.labels = np.array(sim.labels, copy=True)
+            o.n_bulks = s.n_bulks
+        self.bulks = (Bulk * nb)(*[Bulk(*map(float, r)) for r in np.asarray(rows, np.float32)])
         o.nx, o.ny, o.nz = s.nx, s.ny, s.nz
         o.esp, o.gs, o.halfx, o.halfy = s.esp, float(s.grid_size), s.halfx, s.halfy
-        o.n_bulks = s.n_bulks
         o.bulks = C.cast(self.bulks, C.POINTER(Bulk))
         o.labels = self.labels.ctypes.data_as(C.POINTER(C.c_uint8))
         d = s.det

@@ -106,3 +106,55 @@ def test_glass_inclusion_bit_exact(ctx):
     r = O.run_photons(o.o, a_tab(), M.stream_key(5, 0), 0, 8000)
     assert_same(g, r)
     check_totals(g.totals, r.tot)
+
+
+GOLDEN = __import__("pathlib").Path(__file__).parent / "golden" / "ref_inputs"
+
+# three layers: a glass layer (mus = 0, the 2 cm step of :91/:99; mua = 0,
+# since mua * FLT_MAX overflows the weight arithmetic in the reference), an
+# index step inside the slab, and a scattering inclusion straddling two layers
+LAYERS3 = [(1.4, 0.5, 80.0, 0.02, 5.0, 3.0, 0.1, 0.85, 0.05),
+           (1.4, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.9, 0.02),
+           (1.33, 1.0, 120.0, 0.0, 0.0, 0.0, 0.0, 0.9, 0.15)]
+INC3 = (0.05, -0.02, 0.06, 0.05, 1.4, 2.0, 150.0, 0.1, 3.0, 2.0, 0.1, 0.7)
+
+
+def layered3(photons):
+    tmp = (0.0, 0.0, 2, 1, 0.3, 0.3, 0.25, 0, 40, 3000000)
+    return M.Simulation.layered(LAYERS3, INC3, grid_size=20, det=(0.05, 0.05, 16, 16, 0.0, 0.0), photons=photons,
+                                n_above=1.0, n_below=1.0, fhd=True, temporal=tmp)
+
+
+def test_layered_photon_streams_bit_exact(ctx):
+    sim = layered3(20000)
+    print("case: layered slab (3 layers, glass, index step, inclusion), photon streams, 20000 photons", flush=True)
+    g = ctx.run_excitation(sim, seed=31, axyz=True, sq=True)
+    o = O.FromProduct(sim, axyz=True, sq=True)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(31, 0), 0, 20000)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert g.totals["photons"] == 20000 and g.totals["zero_kill"] == 0
+    assert balance(g.totals) == 0
+    assert g.Rd.sum() > 0 and g.Tt.sum() > 0 and g.fhd.sum() > 0 and g.time.sum() > 0
+
+
+def test_layered_sample_mci_lanes_bit_exact(ctx):
+    """The reference's own method-1 input on its 32768-lane geometry (reduced budget)."""
+    sim = M.Simulation.from_mci(str(GOLDEN / "sample.mci"))
+    sim.s.number_of_photons = 40000
+    sim.s.fhd_activated = 1
+    x, a = M.init_rng(M.NUM_THREADS, seed=777)
+    x_cpu = x.copy()
+    print("case: sample.mci (method 1), reference lanes, 40000 photons", flush=True)
+    g = ctx.run_excitation(sim, lanes=(x, a))
+    o = O.FromProduct(sim)
+    r = O.run_lanes(o.o, x_cpu, a, 40000)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert np.array_equal(x, x_cpu), "advanced lane states differ"
+
+
+def test_layered_batch_is_refused(ctx):
+    sim = layered3(1000)
+    with pytest.raises(M.McflError):
+        M.Batch(ctx, [sim], [M.BatchJob(0, 1000, 1)])

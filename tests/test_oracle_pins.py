@@ -8,6 +8,7 @@ slab values, and Chandrasekhar's H-function for a semi-infinite isotropic
 medium.
 """
 import math
+import pathlib
 
 import numpy as np
 import pytest
@@ -16,6 +17,7 @@ import _oracle as O
 import cudamcfl_amd as M
 
 FLT_MAX = float(np.finfo(np.float32).max)
+GOLDEN = pathlib.Path(__file__).parent / "golden" / "ref_inputs"
 
 
 @pytest.fixture(scope="module")
@@ -378,3 +380,82 @@ def test_lanes_require_enough_photons(a_tab):
     x, a = M.init_rng(512, seed=3)
     with pytest.raises(ValueError):
         O.run_lanes(o.o, x, a, 100)
+
+
+# --------------------------------------------- bulk method 1 (layered slab)
+
+FAR = (50.0, 50.0, 50.0, 0.001, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0)  # inclusion no photon reaches
+
+
+def test_layered_forward_beam_exact(a_tab):
+    """g = 1 keeps every photon on the axis (cos theta = 1 exactly), and outside
+    the inclusion the step is the deterministic mutr (CUDAMCFLtransport.cu:99),
+    so every photon follows the same path: steps of mutr clipped at the layer
+    bounds, no absorption at the step that changes layer, w -= rn(mua mutr w)
+    otherwise.  Walk that path in float32 and compare the totals exactly."""
+    f32 = np.float32
+    layers = [(1.0, 0.3, 40.0, 0, 0, 0, 0, 1.0, 0.05), (1.0, 0.8, 25.0, 0, 0, 0, 0, 1.0, 0.03)]
+    sim = M.Simulation.layered(layers, FAR, grid_size=20, det=(0.05, 0.05, 8, 8, 0.0, 0.0), photons=300)
+    o = O.FromProduct(sim)
+    r = O.run_photons(o.o, a_tab, M.stream_key(3, 0), 0, 300)
+    s = sim.s
+    zmin = [f32(s.layers[i].z_min) for i in range(4)]
+    zmax = [f32(s.layers[i].z_max) for i in range(4)]
+    mutr = [f32(s.layers[i].mutr) for i in range(4)]
+    mua_mutr = [f32(f32(s.layers[i].mua) * f32(s.layers[i].mutr)) for i in range(4)]
+    z, L, w, steps, absorbed = f32(0), 1, int(s.start_weight), 0, 0
+    while True:
+        steps += 1
+        st = mutr[L]
+        nl = L
+        if f32(z + st) >= zmax[L]:
+            nl += 1
+            st = f32(zmax[L] - z)
+        z = f32(z + st)
+        if nl != L:
+            L = nl
+            if L > 2:
+                break
+            continue
+        dw = int(np.rint(f32(mua_mutr[L] * f32(w))))
+        w -= dw
+        absorbed += dw
+    assert w > 429497  # the roulette never draws on this path
+    t = r.tot
+    assert t["T"] == 300 * w and t["R"] == 0
+    assert t["A"] == 300 * absorbed and t["steps"] == 300 * steps
+    assert t["launched_w"] == 300 * int(s.start_weight)
+
+
+def test_layered_slab_van_de_hulst(a_tab):
+    """An inclusion that contains the whole slab makes method 1 sample its
+    steps (:87-91) with the inclusion's optics; matched indices at both
+    surfaces: van de Hulst's slab, mua=10, mus=90 /cm, g=0.75, d=0.02 cm,
+    Rd = 0.09739, Tt = 0.66096."""
+    layers = [(1.0, 1.0, 10.0, 0, 0, 0, 0, 0.5, 0.02)]
+    inc = (0.0, 0.0, 0.01, 100.0, 1.0, 10.0, 90.0, 0.0, 0.0, 0.0, 0.0, 0.75)
+    sim = M.Simulation.layered(layers, inc, grid_size=50, photons=10)
+    o = O.FromProduct(sim)
+    N = 300000
+    r = O.run_photons(o.o, a_tab, 2025, 0, N)
+    W = N * 0xFFFFFFFF
+    t = r.tot
+    assert abs(t["R"] / W - 0.09739) < 0.0025
+    assert abs(t["T"] / W - 0.66096) < 0.003
+
+
+def test_layered_sample_mci_energy_identity(a_tab):
+    """The reference's own method-1 input (index step 1.33/1.0, one inclusion,
+    temporal detectors): launched + roulette gain - roulette loss = R + T + A
+    exactly in integers, and no weight is dropped at zero absorption."""
+    sim = M.Simulation.from_mci(str(GOLDEN / "sample.mci"))
+    sim.s.number_of_photons = 3000
+    sim.s.fhd_activated = 1
+    o = O.FromProduct(sim)
+    assert o.o.layered == 1 and o.o.start_layer == 1
+    r = O.run_photons(o.o, a_tab, 11, 0, 3000)
+    t = r.tot
+    assert t["photons"] == 3000 and t["zero_kill"] == 0
+    assert t["launched_w"] + t["roul_gain"] - t["roul_loss"] == t["R"] + t["T"] + t["A"]
+    assert 0 < t["R"] and 0 < t["A"]
+    assert r.fhd.sum() > 0
