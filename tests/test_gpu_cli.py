@@ -38,3 +38,37 @@ def test_cli_sample_mci(tmp_path, ctx):
     assert M.lib().mcfl_write_simulation_results(C.byref(sim.s), C.byref(res.t), 0.0) == 0
     for suffix in ("_Refl.dat", "_Trans.dat", "_time.dat"):
         assert (tmp_path / (out + suffix)).read_bytes() == (api / (out + suffix)).read_bytes(), suffix
+
+
+def test_cli_several_files_equal_single_runs(tmp_path):
+    """`mcfl a.mci b.mci c.mci -S5` (two files on one medium, one on another,
+    temporal detectors, fhd and fluorescence) writes, for every file, the same
+    image, time and PHD files as `mcfl <file> -S5` run alone."""
+    from media import MATS3, small_labels
+    from cudamcfl_amd.media import write_bulk, write_mci
+    lab_a, lab_b = small_labels(20, 20, 10, 0), small_labels(16, 16, 8, 1)
+    mats = [[m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7]] for m in MATS3]
+    tmp = (0.0, 0.0, 2, 1, 0.4, 0.4, 0.3, 0, 50, 1000000)
+    for d in ("many", "one"):
+        (tmp_path / d).mkdir()
+        write_bulk(tmp_path / d / "a.bulk", lab_a)
+        write_bulk(tmp_path / d / "b.bulk", lab_b)
+        for name, bulk, lab, src in (("f1", "a.bulk", lab_a, (0.0, 0.0, 0.0)), ("f2", "a.bulk", lab_a, (0.3, -0.2, 0.0)),
+                                     ("f3", "b.bulk", lab_b, (0.1, 0.1, 0.0))):
+            nz, ny, nx = lab.shape
+            write_mci(tmp_path / d / f"{name}.mci", name, bulk, mats, esp=nz / 10, grid_size=10, photons=40000,
+                      photons_per_voxel=33000, do_fl=1, det=(0.1, 0.1, nx, ny, 0.0, 0.0), source=src, n_below=1.33,
+                      dims=(nx, ny, nz), temporal=tmp)
+    print("case: mcfl CLI, 3 files in one batch vs alone, -S5 -F9", flush=True)
+    r = subprocess.run([str(EXE), "f1.mci", "f2.mci", "f3.mci", "-S5", "-F9"], cwd=tmp_path / "many",
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "in one batch" in r.stdout
+    for name in ("f1", "f2", "f3"):
+        r1 = subprocess.run([str(EXE), f"{name}.mci", "-S5", "-F9"], cwd=tmp_path / "one", capture_output=True,
+                            text=True, timeout=300)
+        assert r1.returncode == 0, r1.stdout[-3000:] + r1.stderr[-3000:]
+        for suffix in ("_Refl.dat", "_Trans.dat", "_time.dat", "_PHD-Ascii.dat", "_FlR.dat", "_FlT.dat"):
+            a = (tmp_path / "many" / (name + suffix)).read_bytes()
+            b = (tmp_path / "one" / (name + suffix)).read_bytes()
+            assert a == b, (name, suffix)
