@@ -216,3 +216,36 @@ def test_cli_rejects_unknown_argument(monkeypatch):
     exe = os.path.join(M.BUILD_DIR, "mcfl")
     p = subprocess.run([exe, "sample2.mci", "-Z"], capture_output=True, text=True)
     assert p.returncode == 1 and "Unknown argument -Z!" in p.stdout
+
+
+def test_output_file_readers_round_trip(tmp_path):
+    """Files written by the C ABI's writers read back to the written values."""
+    from cudamcfl_amd.files import read_image, read_phd, read_time
+    sim = M.Simulation.voxel(np.ones((3, 4, 5), np.uint8), [[1.4, 0.1, 10.0, 0, 0, 0, 0, 0.9]], esp=0.3,
+                             grid_size=10, det=(0.1, 0.2, 5, 4, 0.0, 0.0), photons=1000,
+                             temporal=(0.0, 0.0, 2, 1, 0.5, 0.5, 0.3, 0, 3, 1000), outp=str(tmp_path / "rt"))
+    s = sim.s
+    src = tmp_path / "rt.mci"
+    src.write_text("# input echo\n")
+    s.inp_filename = str(src).encode()
+    s.begin, s.end = 0, 0
+    rng = np.random.default_rng(5)
+    Fx = rng.random(sim.nvox)
+    assert M.lib().mcfl_write_phd(C.byref(s), Fx.ctypes.data_as(M.F64P)) == 0
+    got = read_phd(tmp_path / "rt_PHD-Ascii.dat")
+    assert got.shape == (3, 4, 5)
+    assert np.allclose(got.reshape(-1), Fx, rtol=1e-10, atol=0)
+    assert read_phd(tmp_path / "rt_PHD-Ascii.dat", reference_axes=True).shape == (5, 4, 3)
+    r = M.ExcitationResult(sim, fhd=False)
+    r.Rd[:] = rng.integers(0, 2 ** 40, r.Rd.size, dtype=np.uint64)
+    r.Tt[:] = rng.integers(0, 2 ** 40, r.Tt.size, dtype=np.uint64)
+    r.time[:] = rng.integers(0, 2 ** 40, r.time.size, dtype=np.uint64)
+    assert M.lib().mcfl_write_simulation_results(C.byref(s), C.byref(r.t), 0.5) == 0
+    scale = float(0xFFFFFFFF) * 1000.0
+    area = float(np.float32(0.1)) * float(np.float32(0.2))
+    R = read_image(tmp_path / "rt_Refl.dat")
+    assert R.shape == (4, 5)
+    assert np.allclose(R.reshape(-1), r.Rd[:20].astype(np.float64) / (scale * area), rtol=1e-6)
+    pos, hist = read_time(tmp_path / "rt_time.dat")
+    assert pos.shape == (2, 2) and hist.shape == (2, 3)
+    assert np.allclose(hist[1], r.time[[1, 3, 5]].astype(np.float64) / scale, rtol=1e-6)
