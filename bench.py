@@ -118,6 +118,10 @@ def bench_scan(ctx):
     b = M.Batch(ctx, [sim], jobs)
     b.run()
     t0 = time.perf_counter()
+    tune = b.autotune(1.0 / 16)
+    tune["wall_s"] = time.perf_counter() - t0
+    print(f"scan autotune: {tune}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
     kms = b.run()
     wall_b = time.perf_counter() - t0
     print(f"scan batched: {wall_b:.2f} s wall, {kms:.0f} ms device", file=sys.stderr, flush=True)
@@ -150,7 +154,7 @@ def bench_scan(ctx):
            "photons": total, "batched_photons_per_s": total / wall_b, "batched_device_ms": kms,
            "plain_photons_per_s": total / plain_wall, "plain_device_ms": plain_ms,
            "speedup_wall": plain_wall / wall_b, "speedup_device": plain_ms / kms,
-           "images_identical": same, "rse": rse_stats(jobs),
+           "images_identical": same, "rse": rse_stats(jobs), "autotune": tune,
            "plain_reference_lanes": {"lanes": M.NUM_THREADS, "photons_per_s": total / lanes_wall,
                                      "device_ms": lanes_ms, "speedup_wall": lanes_wall / wall_b,
                                      "rse": rse_stats(lane_res)}}
@@ -168,6 +172,10 @@ def bench_hetero(ctx):
     total = sum(j[4] for j in spec)
     jobs = [M.BatchJob(m, n, seed, source=src, src_kind=kind, src_r=r) for (m, kind, r, src, n, seed) in spec]
     b = M.Batch(ctx, media, jobs)
+    t0 = time.perf_counter()
+    tune = b.autotune(1.0 / 16)
+    tune["wall_s"] = time.perf_counter() - t0
+    print(f"hetero autotune: {tune}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     kms = b.run()
     wall_b = time.perf_counter() - t0
@@ -187,7 +195,7 @@ def bench_hetero(ctx):
                         "budgets log-uniform [1e5,1e8] (seed 11)", "photons": int(total),
             "batched_photons_per_s": total / wall_b, "batched_device_ms": kms,
             "plain_photons_per_s": total / plain_wall, "plain_device_ms": plain_ms,
-            "speedup_wall": plain_wall / wall_b, "images_identical": same}
+            "speedup_wall": plain_wall / wall_b, "images_identical": same, "autotune": tune}
 
 
 def bench_serve(ctx, rate=3.0, duration=12.0, epoch=4_000_000):

@@ -467,6 +467,7 @@ def _bind_engine(L):
     L.mcfl_batch_create.argtypes = [C.c_void_p, SP, C.c_int, C.POINTER(Job), C.c_int, C.POINTER(C.c_void_p)]
     L.mcfl_batch_run.argtypes = [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_double)]
     L.mcfl_batch_destroy.argtypes = [C.c_void_p]
+    L.mcfl_batch_autotune.argtypes = [C.c_void_p, C.c_double, C.POINTER(Tune)]
     L.mcfl_batch_destroy.restype = None
     L.mcfl_run_batch.argtypes = [C.c_void_p, SP, C.c_int, C.POINTER(Job), C.c_int, C.POINTER(C.c_double)]
     L.mcfl_server_create.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]
@@ -483,7 +484,8 @@ def _bind_engine(L):
     L._engine = True
 
 
-EXPORTS += ["mcfl_batch_create", "mcfl_batch_run", "mcfl_batch_destroy", "mcfl_run_batch", "mcfl_server_create",
+EXPORTS += ["mcfl_batch_create", "mcfl_batch_run", "mcfl_batch_destroy", "mcfl_batch_autotune", "mcfl_run_batch",
+            "mcfl_server_create",
             "mcfl_server_add_medium", "mcfl_server_submit", "mcfl_server_step", "mcfl_server_collect",
             "mcfl_server_stats", "mcfl_server_counters", "mcfl_server_destroy"]
 
@@ -517,6 +519,12 @@ class BatchJob:
         return j
 
 
+class Tune(C.Structure):
+    _fields_ = [("tile_side", C.c_int32), ("tile_voxels", C.c_uint32), ("refill_min", C.c_uint32),
+                ("chunk_max", C.c_uint32), ("trial_ms", C.c_double), ("default_ms", C.c_double),
+                ("trials", C.c_int32)]
+
+
 class Batch:
     """mcfl_batch: media + jobs captured once in a HIP graph, replayed by run()."""
 
@@ -541,6 +549,12 @@ class Batch:
             j.totals = {n: int(jj.totals[i]) for i, n in enumerate(TOT_NAMES)}
             j.fl_unit = jj.fl_unit
         return ms.value
+
+    def autotune(self, fraction=1.0 / 16):
+        """mcfl_batch_autotune: pick the launch settings on a scaled-down copy of the batch."""
+        t = Tune()
+        check(lib().mcfl_batch_autotune(self.h, float(fraction), C.byref(t)), "mcfl_batch_autotune")
+        return {n: getattr(t, n) for n, _ in Tune._fields_}
 
     def close(self):
         if self.h:

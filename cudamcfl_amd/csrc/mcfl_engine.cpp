@@ -321,6 +321,34 @@ struct mcfl_batch {
   hipGraphExec_t exec = nullptr;
 };
 
+namespace {
+// (Re)captures the batch: clear every tally and the work counter, the
+// transport launch with b->A, the copy reductions.
+mcfl_status capture(mcfl_batch* b) {
+  mcfl_ctx* c = b->c;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  MCFL_HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  mcfl_status st = MCFL_OK;
+  const size_t n_jobs = b->spec.size();
+  for (size_t j = 0; j < n_jobs && st == MCFL_OK; ++j)
+    st = clear_tallies(*b->media[b->spec[j].medium], b->tal[j].get(), c->stream);
+  if (st == MCFL_OK && mcfl_fill_u64(b->A.counter, 1, c->stream) != 0) st = MCFL_ERR_DEVICE;
+  if (st == MCFL_OK && b->A.total > 0 &&
+      mcfl_transport_launch(&b->A, b->temporal, b->perjob, b->nblocks, c->stream) != 0)
+    st = MCFL_ERR_DEVICE;
+  for (size_t j = 0; j < n_jobs && st == MCFL_OK; ++j)
+    if (reduce_tallies(*b->media[b->spec[j].medium], b->tal[j].get(), c->stream) != 0) st = MCFL_ERR_DEVICE;
+  hipError_t e = hipStreamEndCapture(c->stream, &b->graph);
+  if (st != MCFL_OK) return st;
+  if (e != hipSuccess) return MCFL_ERR_DEVICE;
+  MCFL_HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return MCFL_OK;
+}
+}  // namespace
+
 extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* media, int n_media, const mcfl_job* jobs,
                                          int n_jobs, mcfl_batch** out) {
   if (!c || !media || n_media <= 0 || !jobs || n_jobs <= 0 || !out) return MCFL_ERR_ARG;
@@ -375,22 +403,109 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
   b->nblocks = grid_blocks(c, b->A, b->temporal, b->perjob);
   b->A.n_waves = (uint32_t)b->nblocks * (MCFL_WG / 64u);
   MCFL_HIPCHK(hipStreamSynchronize(c->stream));
-
-  // Capture: clear every tally and the work counter, then the transport launch.
-  MCFL_HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  mcfl_status st = MCFL_OK;
-  for (int j = 0; j < n_jobs && st == MCFL_OK; ++j) st = clear_tallies(*b->media[jobs[j].medium], b->tal[j].get(), c->stream);
-  if (st == MCFL_OK && mcfl_fill_u64(b->A.counter, 1, c->stream) != 0) st = MCFL_ERR_DEVICE;
-  if (st == MCFL_OK && b->A.total > 0 &&
-      mcfl_transport_launch(&b->A, b->temporal, b->perjob, b->nblocks, c->stream) != 0)
-    st = MCFL_ERR_DEVICE;
-  for (int j = 0; j < n_jobs && st == MCFL_OK; ++j)
-    if (reduce_tallies(*b->media[jobs[j].medium], b->tal[j].get(), c->stream) != 0) st = MCFL_ERR_DEVICE;
-  hipError_t e = hipStreamEndCapture(c->stream, &b->graph);
-  if (st != MCFL_OK) return st;
-  if (e != hipSuccess) return MCFL_ERR_DEVICE;
-  MCFL_HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  MCFL_STCHK(capture(b.get()));
   *out = b.release();
+  return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_tune* out) {
+  if (!b || !(fraction > 0.0) || fraction > 1.0) return MCFL_ERR_ARG;
+  mcfl_ctx* c = b->c;
+  if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
+  // the scaled-down copy of the batch: every job keeps its source, medium and
+  // tallies, with fraction of its budget (at least 4096 photons)
+  std::vector<JobDev> jd = b->jd;
+  uint64_t begin = 0;
+  for (JobDev& J : jd) {
+    const uint64_t n = std::max<uint64_t>(std::min<uint64_t>(J.count, 4096), (uint64_t)((double)J.count * fraction));
+    J.begin = begin;
+    J.count = n;
+    begin += n;
+  }
+  DevBuf d_trial;
+  MCFL_STCHK(ensure(d_trial, jd.size() * sizeof(JobDev)));
+  MCFL_HIPCHK(hipMemcpy(d_trial.p, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice));
+  int fhd_job = -1;
+  for (size_t j = 0; j < b->jd.size(); ++j)
+    if (b->jd[j].fhd != nullptr) {
+      fhd_job = (int)j;
+      break;
+    }
+  struct Cand {
+    int side;
+    uint32_t refill, chunk;
+  };
+  std::vector<Cand> cands;
+  const int sides_fhd[] = {0, 16, 32};
+  const uint32_t refills[] = {4, 8}, chunks[] = {1024, 256};
+  for (int sd : sides_fhd) {
+    if (fhd_job < 0 && sd != 0) continue;  // the tile side only matters with an fhd tally
+    for (uint32_t r : refills)
+      for (uint32_t k : chunks) cands.push_back({sd, r, k});
+  }
+  auto setup = [&](const Cand& cd, LaunchArgs& T, int* nblocks) {
+    T = b->A;
+    T.tile_n = 0;
+    if (fhd_job >= 0) {
+      const mcfl_job& sp = b->spec[fhd_job];
+      const Medium& m = *b->media[sp.medium];
+      MediumDev md;
+      fill_medium(&m.sim, m.d_labels.as<const uint8_t>(), &md);
+      choose_tile(c, T, md, sp.xi, sp.yi, (uint32_t)fhd_job, b->temporal, b->perjob, cd.side);
+    }
+    T.refill_min = cd.refill;
+    T.chunk_max = cd.chunk;
+    *nblocks = grid_blocks(c, T, b->temporal, b->perjob);
+    T.n_waves = (uint32_t)*nblocks * (MCFL_WG / 64u);
+  };
+  auto trial = [&](const Cand& cd, double* ms) -> mcfl_status {
+    LaunchArgs T;
+    int nblocks = 0;
+    setup(cd, T, &nblocks);
+    T.jobs = d_trial.as<const JobDev>();
+    T.total = begin;
+    MCFL_HIPCHK(hipMemsetAsync(T.counter, 0, 8, c->stream));
+    MCFL_HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (mcfl_transport_launch(&T, b->temporal, b->perjob, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
+    MCFL_HIPCHK(hipEventRecord(c->ev1, c->stream));
+    MCFL_HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    MCFL_HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return MCFL_OK;
+  };
+  double warm = 0.0, best = 1e300, dflt = 0.0;
+  MCFL_STCHK(trial(cands[0], &warm));  // untimed warm-up of the default
+  size_t bi = 0;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    double t0 = 0.0, t1 = 0.0;
+    MCFL_STCHK(trial(cands[i], &t0));
+    MCFL_STCHK(trial(cands[i], &t1));
+    const double t = std::min(t0, t1);
+    if (i == 0) dflt = t;
+    if (t < best) {
+      best = t;
+      bi = i;
+    }
+  }
+  release(d_trial);
+  // keep the fastest settings and capture the batch with them (trial tallies
+  // are cleared by the graph itself)
+  LaunchArgs T;
+  int nblocks = 0;
+  setup(cands[bi], T, &nblocks);
+  b->A = T;
+  b->nblocks = nblocks;
+  MCFL_STCHK(capture(b));
+  if (out) {
+    out->tile_side = fhd_job >= 0 ? (int32_t)b->A.tnx : 0;
+    out->tile_voxels = b->A.tile_n;
+    out->refill_min = b->A.refill_min;
+    out->chunk_max = b->A.chunk_max;
+    out->trial_ms = best;
+    out->default_ms = dflt;
+    out->trials = (int32_t)(2 * cands.size() + 1);
+  }
   return MCFL_OK;
 }
 

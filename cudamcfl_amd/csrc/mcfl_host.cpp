@@ -198,7 +198,7 @@ int grid_blocks(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob
 }
 
 void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi, float yi, uint32_t job, int temporal,
-                 int perjob) {
+                 int perjob, int side_req) {
   A.tile_n = 0;
   const uint32_t nslots = A.work == MCFL_WORK_JOBS ? A.njobs : 1u;
   const int layered = A.med.n_layers ? 1 : 0;
@@ -210,7 +210,8 @@ void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi,
   const uint32_t cap = (uint32_t)std::min<size_t>((per_wg - other) / 8, 1u << 15);
   // box 2r x 2r x r around the source (the photon cloud is about as deep as it is wide)
   int side = std::max(2, (int)std::lround(2.0 * std::cbrt(cap / 4.0)));
-  if (c->tile_side > 0) side = std::max(2, std::min(c->tile_side, (int)std::sqrt((double)cap)));
+  const int want = side_req > 0 ? side_req : c->tile_side;
+  if (want > 0) side = std::max(2, std::min(want, (int)std::sqrt((double)cap)));
   const uint32_t tnx = (uint32_t)std::min(m.nx, side), tny = (uint32_t)std::min(m.ny, side);
   const uint32_t tnz = std::min<uint32_t>((uint32_t)m.nz, cap / (tnx * tny));
   if (tnz == 0) return;

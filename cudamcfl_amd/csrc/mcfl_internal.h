@@ -73,8 +73,9 @@ mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob
 int grid_blocks(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob);
 // Chooses the fhd LDS tile of job `job` (box around its source voxel) within
 // the LDS left free at the kernel's register-limited occupancy.
+// side > 0 forces the tile's lateral size (clamped to the budget).
 void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi, float yi, uint32_t job, int temporal,
-                 int perjob);
+                 int perjob, int side = 0);
 }  // namespace mcfl_detail
 
 #define MCFL_HIPCHK(expr)                           \

@@ -257,6 +257,23 @@ mcfl_status mcfl_batch_create(mcfl_ctx* ctx, const mcfl_sim* const* media, int n
 /* Replays the graph with jobs[i].seed and copies each job's outputs into jobs[i]. */
 mcfl_status mcfl_batch_run(mcfl_batch* b, mcfl_job* jobs, double* kernel_ms);
 void mcfl_batch_destroy(mcfl_batch* b);
+
+/* Launch settings of a batch picked by mcfl_batch_autotune. */
+typedef struct mcfl_tune {
+  int32_t tile_side;    /* lateral size of the fhd LDS tile in voxels (0: no fhd tally) */
+  uint32_t tile_voxels; /* entries of the tile */
+  uint32_t refill_min;  /* idle lanes of a wave that trigger its refill */
+  uint32_t chunk_max;   /* largest photon chunk a wave takes from the work counter */
+  double trial_ms;      /* device time of the chosen settings on the scaled-down batch */
+  double default_ms;    /* the same for the default settings */
+  int32_t trials;       /* trial launches made */
+} mcfl_tune;
+
+/* No counterpart in the reference.  Times candidate launch settings (fhd tile
+ * side, refill threshold, chunk size) on a copy of the batch with every budget
+ * scaled by `fraction` (at least 4096 photons per job), keeps the fastest and
+ * re-captures the graph with it.  Results do not depend on these settings. */
+mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_tune* out);
 /* create + run + destroy */
 mcfl_status mcfl_run_batch(mcfl_ctx* ctx, const mcfl_sim* const* media, int n_media, mcfl_job* jobs, int n_jobs,
                            double* kernel_ms);

@@ -156,3 +156,23 @@ def test_server_admission_by_footprint(ctx):
     with pytest.raises(M.McflError):
         srv.submit(mid, 100, 1, fluorescence=True)  # its footprint exceeds the whole budget
     srv.close()
+
+
+def test_autotuned_batch_equals_default(ctx):
+    """Autotuning changes launch settings only: the replayed batch gives the
+    same integer tallies as before tuning."""
+    sim = medium(temporal=True)
+    jobs = [M.BatchJob(0, 6000, 50 + i, source=s, tallies=M.TALLY_FHD | M.TALLY_SQ | M.TALLY_TEMP)
+            for i, s in enumerate([(0.0, 0.0, 0.0), (0.3, 0.1, 0.0)])]
+    print("case: batch of 2 sources, autotune at fraction 1/4", flush=True)
+    b = M.Batch(ctx, [sim], jobs)
+    b.run()
+    before = [(j.Rd.copy(), j.fhd.copy(), j.time.copy(), dict(j.totals)) for j in jobs]
+    t = b.autotune(0.25)
+    assert t["trials"] > 1 and t["refill_min"] in (4, 8) and t["chunk_max"] in (256, 1024)
+    assert t["tile_side"] > 0 and t["trial_ms"] <= t["default_ms"]
+    b.run()
+    for j, (rd, fhd, tim, tot) in zip(jobs, before):
+        assert np.array_equal(j.Rd, rd) and np.array_equal(j.fhd, fhd) and np.array_equal(j.time, tim)
+        assert j.totals == tot
+    b.close()
