@@ -297,7 +297,7 @@ struct Launcher {
     A->a_mask = kTablePhotons - 1;
     A->jobs = d_jobs.as<const JobDev>();
     A->njobs = (uint32_t)jd.size();
-    A->chunk_max = 1024;
+    A->chunk_max = c->chunk_max;
     A->refill_min = c->refill_min;
     return MCFL_OK;
   }

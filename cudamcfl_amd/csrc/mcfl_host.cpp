@@ -245,7 +245,7 @@ mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob
                             const std::function<int()>& post) {
   const int nblocks = grid_blocks(c, A, temporal, perjob);
   A.n_waves = (uint32_t)nblocks * (MCFL_WG / 64u);
-  if (A.chunk_max == 0) A.chunk_max = 1024;
+  if (A.chunk_max == 0) A.chunk_max = c->chunk_max;
   A.refill_min = c->refill_min;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (mcfl_transport_launch(&A, temporal, perjob, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
@@ -400,6 +400,7 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   if (const char* e = getenv("MCFL_TALLY_COPIES")) c->copies = atoi(e) > 1 ? MCFL_TALLY_COPIES : 1;
   if (const char* e = getenv("MCFL_TILE_SIDE")) c->tile_side = atoi(e);
   if (const char* e = getenv("MCFL_REFILL_MIN")) c->refill_min = (uint32_t)std::min(64, std::max(1, atoi(e)));
+  if (const char* e = getenv("MCFL_CHUNK_MAX")) c->chunk_max = (uint32_t)std::min(1 << 20, std::max(64, atoi(e)));
   snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
   for (int w = 0; w < 3; ++w)
     for (int t = 0; t < 2; ++t)

@@ -59,6 +59,7 @@ struct mcfl_ctx {
   uint32_t copies = MCFL_TALLY_COPIES;  // copies of the per-step volume tallies (1 or MCFL_TALLY_COPIES)
   int tile_side = 0;                    // lateral size of the fhd LDS tile, 0: from the LDS budget (env MCFL_TILE_SIDE)
   uint32_t refill_min = 4;              // idle lanes that trigger a wave's refill (env MCFL_REFILL_MIN)
+  uint32_t chunk_max = 1024;            // largest photon chunk a wave takes from the counter (env MCFL_CHUNK_MAX)
   char name[256] = {};
   uint32_t* d_atab = nullptr;
   std::vector<uint32_t> atab;

@@ -144,24 +144,27 @@ __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint3
   float t = mt.onemg2 / mcfl_fma(mt.twog, xi, mt.onemg);
   float cost = mcfl_fma(-t, t, mt.onepg2) * mt.inv2g;
   if (mt.twog == 0.0f) cost = 2.0f * mcfl_rand_co(&x, a) - 1.0f;
-  float sint = mcfl_sqrt_rs(mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f))); /* 0 or >= 2^-23 */
+  const float s2 = mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)); /* sin^2 theta: 0 or >= 2^-23 */
   float sps, cps;
   mcfl_sincos_turn(mcfl_rand_co(&x, a), &sps, &cps);
   const float cosp = sps, sinp = cps; /* the reference's __sincosf argument order */
   const float dx = p.dx, dy = p.dy, dz = p.dz;
-  const float q = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)); /* sin^2 of the polar angle; 0 or >= 6e-8 */
+  const float q = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)); /* temp^2 = 1 - dz^2: 0 or >= 6e-8 */
   float nx, ny, nz;
-  if (q == 0.0f) { /* the reference's temp == 0 test on sqrt(q) */
+  if (q == 0.0f) { /* the reference's temp == 0 test */
+    const float sint = mcfl_sqrt_rs(s2);
     nx = sint * cosp;
     ny = sint * sinp;
     nz = mcfl_copysign(cost, dz * cost);
   } else {
-    const float itt = mcfl_rsqrt(q); /* 1/temp */
-    const float tt = q * itt;        /* temp */
-    const float stt = sint * itt;
+    /* sint/temp and sint*temp from one reciprocal root of sint^2 temp^2 */
+    const float pq = s2 * q;
+    const float r = pq > 0.0f ? mcfl_rsqrt(pq) : 0.0f;
+    const float stt = s2 * r; /* sint / temp */
+    const float stq = pq * r; /* sint * temp */
     nx = mcfl_fma(stt, dx * dz * cosp - dy * sinp, dx * cost);
     ny = mcfl_fma(stt, dy * dz * cosp + dx * sinp, dy * cost);
-    nz = mcfl_fma(-sint * cosp, tt, dz * cost);
+    nz = mcfl_fma(-stq, cosp, dz * cost);
   }
   const float r = mcfl_renorm(nx * nx + ny * ny + nz * nz);
   p.dx = nx * r;

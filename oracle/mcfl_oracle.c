@@ -173,26 +173,27 @@ static void orc_spin(const mat_t* m, orc_photon* p, uint64_t* x, uint32_t a) {
   float temp = m->onemg2 / mcfl_fma(m->twog, xi, m->onemg);
   float cost = mcfl_fma(-temp, temp, m->onepg2) * m->inv2g;
   if (m->g == 0.0f) cost = 2.0f * mcfl_rand_co(x, a) - 1.0f;
-  float sint = mcfl_sqrt_rs(mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)));
+  float sin2t = mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)); /* sint^2 (:600) */
   float s_psi, c_psi;
   mcfl_sincos_turn(mcfl_rand_co(x, a), &s_psi, &c_psi);
   float cosp = s_psi; /* :603 assigns the sine to cosp */
   float sinp = c_psi;
   float dx = p->dx, dy = p->dy, dz = p->dz;
-  /* temp = sqrt(1 - dz^2) (:605); 1/temp and temp from one reciprocal root */
-  float s2 = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f));
+  float temp2 = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)); /* temp^2, temp = sqrt(1 - dz^2) (:605) */
   float ndx, ndy, ndz;
-  if (s2 == 0.0f) { /* :607-612 normal incidence */
+  if (temp2 == 0.0f) { /* :607-612 normal incidence */
+    float sint = mcfl_sqrt_rs(sin2t);
     ndx = sint * cosp;
     ndy = sint * sinp;
     ndz = mcfl_copysign(cost, dz * cost);
-  } else { /* :613-618 */
-    float inv = mcfl_rsqrt(s2);
-    float tt = s2 * inv;
-    float st = sint * inv;
+  } else { /* :613-618, with sint/temp and sint*temp from one reciprocal root of sint^2 temp^2 */
+    float pq = sin2t * temp2;
+    float r = pq > 0.0f ? mcfl_rsqrt(pq) : 0.0f;
+    float st = sin2t * r;
+    float stq = pq * r;
     ndx = mcfl_fma(st, dx * dz * cosp - dy * sinp, dx * cost);
     ndy = mcfl_fma(st, dy * dz * cosp + dx * sinp, dy * cost);
-    ndz = mcfl_fma(-sint * cosp, tt, dz * cost);
+    ndz = mcfl_fma(-stq, cosp, dz * cost);
   }
   /* :621-624 renormalisation with a reciprocal square root (first order near 1) */
   float r = mcfl_renorm(ndx * ndx + ndy * ndy + ndz * ndz);
