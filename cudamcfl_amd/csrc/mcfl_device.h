@@ -97,6 +97,8 @@ struct JobDev {
   uint32_t mat_base; /* first entry of this job's material table in the launch table */
   uint32_t start_lab; /* layered medium: the layer holding zi */
   uint32_t flags;
+  uint32_t tile;     /* 1: workgroups at home on this job privatise its fhd in an LDS tile */
+  uint32_t tx0, ty0; /* the tile's first voxel in x and y (it starts at z = 0) */
   double exit_scale;
   unsigned long long* Rd;
   unsigned long long* Tt;
@@ -122,7 +124,7 @@ struct LaunchArgs {
   uint32_t n_mats;
   int32_t work; /* MCFL_WORK_* */
   uint64_t total; /* size of the flattened index space */
-  unsigned long long* counter; /* global work counter (zeroed before launch) */
+  unsigned long long* counter; /* one work counter per job (zeroed before launch) */
   const uint32_t* a_tab; /* photon-stream multipliers */
   uint32_t a_mask;       /* table length - 1 (power of two) */
   /* MCFL_WORK_JOBS */
@@ -137,10 +139,9 @@ struct LaunchArgs {
   unsigned long long* xs;
   const uint32_t* as;
   uint64_t lane_q, lane_rem;
-  /* LDS tile of one job's photon hitting density: voxels [t0, t0+tn) per axis */
-  uint32_t tile_n;   /* entries (0: no tile) */
-  uint32_t tile_job; /* the job whose fhd the tile privatises */
-  int32_t tx0, ty0, tz0;
+  /* LDS tile of a workgroup's home job's photon hitting density: voxels
+     [t0, t0+tn) per axis, t0 per job (tz0 = 0) */
+  uint32_t tile_n; /* entries (0: no tile) */
   uint32_t tnx, tny, tnz;
   /* launch geometry */
   uint32_t n_waves;

@@ -281,8 +281,7 @@ struct Launcher {
     if (jd.size() > MCFL_MAX_JOBS) return MCFL_ERR_CAPACITY;
     MCFL_STCHK(ensure(d_jobs, jd.size() * sizeof(JobDev)));
     MCFL_STCHK(ensure(d_mats, mt.mats.size() * sizeof(MatDev)));
-    MCFL_STCHK(ensure(d_counter, 8));
-    MCFL_HIPCHK(hipMemcpyAsync(d_jobs.p, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
+    MCFL_STCHK(ensure(d_counter, 8 * MCFL_MAX_JOBS));
     MCFL_HIPCHK(hipMemcpyAsync(d_mats.p, mt.mats.data(), mt.mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, st));
     memset(A, 0, sizeof *A);
     if (!perjob) fill_medium(&uniform->sim, uniform->d_labels.as<const uint8_t>(), &A->med);
@@ -299,6 +298,11 @@ struct Launcher {
     A->njobs = (uint32_t)jd.size();
     A->chunk_max = c->chunk_max;
     A->refill_min = c->refill_min;
+    return MCFL_OK;
+  }
+  // Job table upload, after the tiles are placed.
+  mcfl_status upload(const std::vector<JobDev>& jd, hipStream_t st) {
+    MCFL_HIPCHK(hipMemcpyAsync(d_jobs.p, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
     return MCFL_OK;
   }
 };
@@ -335,7 +339,7 @@ mcfl_status capture(mcfl_batch* b) {
   const size_t n_jobs = b->spec.size();
   for (size_t j = 0; j < n_jobs && st == MCFL_OK; ++j)
     st = clear_tallies(*b->media[b->spec[j].medium], b->tal[j].get(), c->stream);
-  if (st == MCFL_OK && mcfl_fill_u64(b->A.counter, 1, c->stream) != 0) st = MCFL_ERR_DEVICE;
+  if (st == MCFL_OK && mcfl_fill_u64(b->A.counter, b->A.njobs, c->stream) != 0) st = MCFL_ERR_DEVICE;
   if (st == MCFL_OK && b->A.total > 0 &&
       mcfl_transport_launch(&b->A, b->temporal, b->perjob, b->nblocks, c->stream) != 0)
     st = MCFL_ERR_DEVICE;
@@ -392,14 +396,8 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
     }
   b->perjob = n_used > 1;
   MCFL_STCHK(b->L.prepare(c, b->jd, b->mt, b->perjob, uni, &b->A, c->stream));
-  for (int j = 0; j < n_jobs; ++j)  // LDS fhd tile for the first job tallying fhd
-    if (b->jd[j].fhd != nullptr) {
-      const Medium& m = *b->media[jobs[j].medium];
-      MediumDev md;
-      fill_medium(&m.sim, m.d_labels.as<const uint8_t>(), &md);
-      choose_tile(c, b->A, md, jobs[j].xi, jobs[j].yi, (uint32_t)j, b->temporal, b->perjob);
-      break;
-    }
+  plan_tiles(c, b->A, b->jd.data(), b->jd.size(), b->temporal, b->perjob);  // an LDS fhd tile per fhd job
+  MCFL_STCHK(b->L.upload(b->jd, c->stream));
   b->nblocks = grid_blocks(c, b->A, b->temporal, b->perjob);
   b->A.n_waves = (uint32_t)b->nblocks * (MCFL_WG / 64u);
   MCFL_HIPCHK(hipStreamSynchronize(c->stream));
@@ -424,7 +422,6 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
   }
   DevBuf d_trial;
   MCFL_STCHK(ensure(d_trial, jd.size() * sizeof(JobDev)));
-  MCFL_HIPCHK(hipMemcpy(d_trial.p, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice));
   int fhd_job = -1;
   for (size_t j = 0; j < b->jd.size(); ++j)
     if (b->jd[j].fhd != nullptr) {
@@ -443,16 +440,9 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
     for (uint32_t r : refills)
       for (uint32_t k : chunks) cands.push_back({sd, r, k});
   }
-  auto setup = [&](const Cand& cd, LaunchArgs& T, int* nblocks) {
+  auto setup = [&](const Cand& cd, LaunchArgs& T, std::vector<JobDev>& tj, int* nblocks) {
     T = b->A;
-    T.tile_n = 0;
-    if (fhd_job >= 0) {
-      const mcfl_job& sp = b->spec[fhd_job];
-      const Medium& m = *b->media[sp.medium];
-      MediumDev md;
-      fill_medium(&m.sim, m.d_labels.as<const uint8_t>(), &md);
-      choose_tile(c, T, md, sp.xi, sp.yi, (uint32_t)fhd_job, b->temporal, b->perjob, cd.side);
-    }
+    plan_tiles(c, T, tj.data(), tj.size(), b->temporal, b->perjob, cd.side);
     T.refill_min = cd.refill;
     T.chunk_max = cd.chunk;
     *nblocks = grid_blocks(c, T, b->temporal, b->perjob);
@@ -461,10 +451,11 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
   auto trial = [&](const Cand& cd, double* ms) -> mcfl_status {
     LaunchArgs T;
     int nblocks = 0;
-    setup(cd, T, &nblocks);
+    setup(cd, T, jd, &nblocks);
+    MCFL_HIPCHK(hipMemcpyAsync(d_trial.p, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, c->stream));
     T.jobs = d_trial.as<const JobDev>();
     T.total = begin;
-    MCFL_HIPCHK(hipMemsetAsync(T.counter, 0, 8, c->stream));
+    MCFL_HIPCHK(hipMemsetAsync(T.counter, 0, 8 * (size_t)T.njobs, c->stream));
     MCFL_HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (mcfl_transport_launch(&T, b->temporal, b->perjob, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
     MCFL_HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -493,7 +484,8 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
   // are cleared by the graph itself)
   LaunchArgs T;
   int nblocks = 0;
-  setup(cands[bi], T, &nblocks);
+  setup(cands[bi], T, b->jd, &nblocks);
+  MCFL_STCHK(b->L.upload(b->jd, c->stream));
   b->A = T;
   b->nblocks = nblocks;
   MCFL_STCHK(capture(b));
@@ -765,7 +757,6 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
     const Medium* uni = nullptr;
     bool perjob = false;
     uint64_t begin = 0;
-    int tile_job = -1;
     for (std::unique_ptr<Request>& qp : s->active) {
       Request& q = *qp;
       if ((q.phase != 1 && q.phase != 2) || q.done >= q.r.photons) continue;
@@ -777,7 +768,6 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
       JobDev J;
       fill_job(J, *q.m, t, em ? MCFL_SOURCE_EMISSION : q.r.src_kind, q.r.xi, q.r.yi, q.r.zi, q.r.src_r, q.r.seed,
                begin, take, q.done, mt.base_of(q.m, em));
-      if (J.fhd != nullptr && tile_job < 0) tile_job = (int)jd.size();
       jd.push_back(J);
       E.parts.emplace_back(q.id, take);
       begin += take;
@@ -787,17 +777,11 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
     if (!jd.empty()) {
       LaunchArgs A;
       MCFL_STCHK(E.L.prepare(c, jd, mt, perjob, uni, &A, E.stream));
-      if (tile_job >= 0) {  // LDS fhd tile for the first request in its excitation phase
-        const Request* q = nullptr;
-        for (const std::unique_ptr<Request>& qp : s->active)
-          if (qp->id == E.parts[tile_job].first) q = qp.get();
-        MediumDev md;
-        fill_medium(&q->m->sim, q->m->d_labels.as<const uint8_t>(), &md);
-        choose_tile(c, A, md, q->r.xi, q->r.yi, (uint32_t)tile_job, 0, perjob ? 1 : 0);
-      }
+      plan_tiles(c, A, jd.data(), jd.size(), 0, perjob ? 1 : 0);  // an LDS fhd tile per request tallying fhd
+      MCFL_STCHK(E.L.upload(jd, E.stream));
       const int nblocks = grid_blocks(c, A, 0, perjob ? 1 : 0);
       A.n_waves = (uint32_t)nblocks * (MCFL_WG / 64u);
-      MCFL_HIPCHK(hipMemsetAsync(A.counter, 0, 8, E.stream));
+      MCFL_HIPCHK(hipMemsetAsync(A.counter, 0, 8 * jd.size(), E.stream));
       MCFL_HIPCHK(hipEventRecord(E.e0, E.stream));
       if (mcfl_transport_launch(&A, 0, perjob ? 1 : 0, nblocks, E.stream) != 0) return MCFL_ERR_DEVICE;
       MCFL_HIPCHK(hipEventRecord(E.e1, E.stream));

@@ -197,8 +197,7 @@ int grid_blocks(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob
   return c->n_cu * bpc;
 }
 
-void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi, float yi, uint32_t job, int temporal,
-                 int perjob, int side_req) {
+void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, int temporal, int perjob, int side_req) {
   A.tile_n = 0;
   const uint32_t nslots = A.work == MCFL_WORK_JOBS ? A.njobs : 1u;
   const int layered = A.med.n_layers ? 1 : 0;
@@ -215,16 +214,9 @@ void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi,
   const uint32_t tnx = (uint32_t)std::min(m.nx, side), tny = (uint32_t)std::min(m.ny, side);
   const uint32_t tnz = std::min<uint32_t>((uint32_t)m.nz, cap / (tnx * tny));
   if (tnz == 0) return;
-  auto clampi = [](long v, long lo, long hi) { return v < lo ? lo : (v > hi ? hi : v); };
-  const long ix = clampi((long)std::floor((xi + m.halfx) * m.gs), 0, m.nx - 1);
-  const long iy = clampi((long)std::floor((yi + m.halfy) * m.gs), 0, m.ny - 1);
   A.tnx = tnx;
   A.tny = tny;
   A.tnz = tnz;
-  A.tx0 = (int32_t)clampi(ix - (long)tnx / 2, 0, m.nx - (long)tnx);
-  A.ty0 = (int32_t)clampi(iy - (long)tny / 2, 0, m.ny - (long)tny);
-  A.tz0 = 0;
-  A.tile_job = job;
   A.tile_n = tnx * tny * tnz;
   // keep the register-limited occupancy
   int got = 0;
@@ -239,6 +231,31 @@ void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi,
     A.tnz -= 1;
     A.tile_n = A.tnx * A.tny * A.tnz;
   }
+}
+
+void place_tile(const LaunchArgs& A, const MediumDev& m, float xi, float yi, JobDev& J) {
+  J.tile = 0;
+  J.tx0 = J.ty0 = 0;
+  if (A.tile_n == 0 || A.tnx > (uint32_t)m.nx || A.tny > (uint32_t)m.ny || A.tnz > (uint32_t)m.nz) return;
+  auto clampi = [](long v, long lo, long hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  const long ix = clampi((long)std::floor((xi + m.halfx) * m.gs), 0, m.nx - 1);
+  const long iy = clampi((long)std::floor((yi + m.halfy) * m.gs), 0, m.ny - 1);
+  J.tx0 = (uint32_t)clampi(ix - (long)A.tnx / 2, 0, m.nx - (long)A.tnx);
+  J.ty0 = (uint32_t)clampi(iy - (long)A.tny / 2, 0, m.ny - (long)A.tny);
+  J.tile = 1;
+}
+
+void plan_tiles(const mcfl_ctx* c, LaunchArgs& A, JobDev* jd, size_t n, int temporal, int perjob, int side_req) {
+  A.tile_n = 0;
+  const JobDev* first = nullptr;
+  for (size_t j = 0; j < n; ++j) {
+    jd[j].tile = 0;
+    if (jd[j].fhd != nullptr && !first) first = &jd[j];
+  }
+  if (!first) return;
+  choose_tile(c, A, first->med, temporal, perjob, side_req);
+  for (size_t j = 0; j < n; ++j)
+    if (jd[j].fhd != nullptr) place_tile(A, jd[j].med, jd[j].xi, jd[j].yi, jd[j]);
 }
 
 mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms,
@@ -529,8 +546,7 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     d_tpos = static_cast<const float*>(c->tpos.p);
   }
   fill_det(s, &J.det, d_tpos);
-  STCHK(ensure(c->jobs, sizeof(JobDev)));
-  HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
+  STCHK(ensure(c->jobs, sizeof(JobDev)));  // uploaded below, once its tile is placed
 
   A.mats = static_cast<const MatDev*>(c->mats.p);
   A.n_mats = (uint32_t)mats.size();
@@ -555,7 +571,11 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     A.work = MCFL_WORK_JOBS;
     A.total = s->number_of_photons;
   }
-  if (fhd_on) choose_tile(c, A, A.med, s->xi, s->yi, 0, temporal, 0);
+  if (fhd_on) {
+    choose_tile(c, A, A.med, temporal, 0);
+    place_tile(A, A.med, s->xi, s->yi, J);
+  }
+  HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
   auto reduce = [&]() -> int {
     int e = 0;
     if (fhd_on) e |= mcfl_reduce_copies(c->fhd.as<unsigned long long>(), nv, copies, c->stream);

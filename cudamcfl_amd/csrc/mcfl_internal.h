@@ -72,11 +72,17 @@ namespace mcfl_detail {
 mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms,
                             const std::function<int()>& post = {});
 int grid_blocks(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob);
-// Chooses the fhd LDS tile of job `job` (box around its source voxel) within
-// the LDS left free at the kernel's register-limited occupancy.
-// side > 0 forces the tile's lateral size (clamped to the budget).
-void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, float xi, float yi, uint32_t job, int temporal,
-                 int perjob, int side = 0);
+// Chooses the dimensions of the fhd LDS tile (A.tnx, tny, tnz, tile_n) for a
+// medium of m's size within the LDS left free at the kernel's
+// register-limited occupancy.  side > 0 forces the tile's lateral size
+// (clamped to the budget).
+void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, int temporal, int perjob, int side = 0);
+// Places job J's tile (dimensions from A) around its source voxel in medium m;
+// J.tile = 0 when the tile does not fit that medium.
+void place_tile(const LaunchArgs& A, const MediumDev& m, float xi, float yi, JobDev& J);
+// Tiles of a job launch: dimensions for the first job tallying fhd, then one
+// tile per fhd job (the jobs' med fields hold their media).
+void plan_tiles(const mcfl_ctx* c, LaunchArgs& A, JobDev* jd, size_t n, int temporal, int perjob, int side = 0);
 }  // namespace mcfl_detail
 
 #define MCFL_HIPCHK(expr)                           \

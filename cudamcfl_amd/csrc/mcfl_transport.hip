@@ -305,9 +305,9 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-__device__ __forceinline__ uint32_t find_job(const LaunchArgs& A, uint64_t g, uint32_t hint) {
+/* The job whose flattened index range holds g (jobs in begin order). */
+__device__ __forceinline__ uint32_t job_at(const LaunchArgs& A, uint64_t g) {
   const JobDev* J = A.jobs;
-  if (g >= J[hint].begin && g - J[hint].begin < J[hint].count) return hint;
   uint32_t lo = 0, hi = A.njobs; /* last job with begin <= g */
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
@@ -336,6 +336,26 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
   if (LAYERED)
     for (uint32_t i = tid; i < 2u * (A.med.n_layers + 2u); i += MCFL_WG) szb[i] = A.med.zb[i];
   for (uint32_t i = tid; i < nslots * MCFL_NTOT + A.tile_n; i += MCFL_WG) lds[i] = 0ull;
+  /* Work: each job has its own counter over its photons [0, count).  A
+     workgroup starts on its home job (jobs get workgroups in proportion to
+     their share of the photons), whose fhd it privatises in its LDS tile,
+     and moves on to the other jobs when its home job is exhausted.  The
+     workgroup's constants live in LDS (read on voxel changes and refills, so
+     they hold no registers): home job, tile job (~0: no tile), tile origin. */
+  __shared__ uint32_t swg[4];
+  if (tid == 0) {
+    uint32_t home = 0;
+    if (WORK == MCFL_WORK_JOBS && A.njobs > 1) {
+      const uint64_t gpos = (uint64_t)(((double)blockIdx.x + 0.5) / (double)gridDim.x * (double)A.total);
+      home = job_at(A, gpos < A.total ? gpos : A.total - 1ull);
+    }
+    const JobDev& HJ = A.jobs[home];
+    const bool tiled = A.tile_n != 0u && HJ.tile != 0u;
+    swg[0] = home;
+    swg[1] = tiled ? home : 0xFFFFFFFFu;
+    swg[2] = tiled ? HJ.tx0 : 0u;
+    swg[3] = tiled ? HJ.ty0 : 0u;
+  }
   __syncthreads();
 
   /* the medium: launch-uniform, or the current photon's job's (PERJOB) */
@@ -365,8 +385,9 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
   uint32_t lane_item = 0;
   uint64_t lane_left = 0;
 
-  /* wave-uniform chunk of the index space */
-  uint64_t pool_next = 0, pool_end = 0, last_base = 0;
+  /* wave-uniform chunk of the current job's photons: [pool_next, pool_next + pool_left) */
+  uint32_t cur = swg[0], pool_left = 0;
+  uint64_t pool_next = 0;
   bool pool_dead = false;
 
   if (WORK == MCFL_WORK_VOXELS) {
@@ -385,25 +406,31 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
     if (needmask != 0ull && (uint32_t)__popcll(needmask) < A.refill_min && __ballot(alive) != 0ull && !pool_dead)
       needmask = 0ull;
     while (needmask != 0ull) {
-      if (pool_next == pool_end) {
+      if (pool_left == 0u) {
         if (pool_dead) break;
-        uint64_t rem = A.total > last_base ? A.total - last_base : 0ull;
+        const uint64_t cnt = WORK == MCFL_WORK_JOBS ? A.jobs[cur].count : A.total;
+        /* guided: the job's remainder (beyond the wave's last chunk) over
+           four times the launch's waves, within [64, chunk_max] */
+        const uint64_t rem = cnt > pool_next ? cnt - pool_next : 0ull;
         uint64_t chunk = rem / ((uint64_t)A.n_waves * 4ull);
         chunk = chunk < 64ull ? 64ull : (chunk > A.chunk_max ? (uint64_t)A.chunk_max : chunk);
         unsigned long long base = 0ull;
-        if (lane == 0) base = atomicAdd(A.counter, (unsigned long long)chunk);
+        if (lane == 0) base = atomicAdd(A.counter + cur, (unsigned long long)chunk);
         base = __shfl(base, 0);
-        if (base >= A.total) {
-          pool_dead = true;
-          break;
+        if (base >= cnt) { /* this job is exhausted: the next one, until the home job comes round again */
+          cur = (cur + 1u == nslots) ? 0u : cur + 1u;
+          pool_next = 0;
+          if (cur == swg[0]) {
+            pool_dead = true;
+            break;
+          }
+          continue;
         }
-        last_base = base;
         pool_next = base;
-        pool_end = (base + chunk < A.total) ? base + chunk : A.total;
+        pool_left = (uint32_t)(base + chunk < cnt ? chunk : cnt - base);
       }
       const uint32_t n = (uint32_t)__popcll(needmask);
-      const uint64_t avail = pool_end - pool_next;
-      const uint32_t take = avail < (uint64_t)n ? (uint32_t)avail : n;
+      const uint32_t take = pool_left < n ? pool_left : n;
       const uint32_t rank = lane_rank(needmask);
       if (need && rank < take) {
         const uint64_t g = pool_next + rank;
@@ -412,9 +439,9 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
         accA = 0;
         iters = 0;
         if (WORK == MCFL_WORK_JOBS) {
-          job = find_job(A, g, job);
+          job = cur;
           const JobDev& J = A.jobs[job];
-          const uint64_t k = g - J.begin + J.k0;
+          const uint64_t k = g + J.k0;
           a = A.a_tab[k & A.a_mask];
           x = mcfl_photon_state(J.key, k, a);
           mbase = J.mat_base;
@@ -467,6 +494,7 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
         lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
       }
       pool_next += take;
+      pool_left -= take;
       needmask = __ballot(need);
     }
     if (need && pool_dead) retired = true;
@@ -560,8 +588,8 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
         fv = vi;
         facc = 0ull;
         ft = 0xFFFFFFFFu;
-        if (A.tile_n != 0u && job == A.tile_job) { /* inside the LDS tile? */
-          const uint32_t lx = cl.ix - (uint32_t)A.tx0, ly = cl.iy - (uint32_t)A.ty0, lz = cl.iz - (uint32_t)A.tz0;
+        if (job == swg[1]) { /* inside the LDS tile of the workgroup's home job? */
+          const uint32_t lx = cl.ix - swg[2], ly = cl.iy - swg[3], lz = cl.iz;
           if (lx < A.tnx && ly < A.tny && lz < A.tnz) ft = lx + __umul24(A.tnx, ly + __umul24(A.tny, lz));
         }
       }
@@ -652,16 +680,16 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
     const unsigned long long v = stot[i];
     if (v != 0ull) atomic_add_u64(A.jobs[i / MCFL_NTOT].tot + (i % MCFL_NTOT), v);
   }
-  if (A.tile_n != 0u) {
-    const JobDev& J = A.jobs[A.tile_job];
-    unsigned long long* dst = J.fhd + xcc * J.copy_stride;
+  if (swg[1] != 0xFFFFFFFFu) {
+    const JobDev& HJ = A.jobs[swg[1]];
+    unsigned long long* dst = HJ.fhd + xcc * HJ.copy_stride;
+    const MediumDev& tm = PERJOB ? HJ.med : A.med; /* the home job's medium, not the lane's last one */
+    const uint32_t tx0 = swg[2], ty0 = swg[3];
     for (uint32_t i = tid; i < A.tile_n; i += MCFL_WG) {
       const unsigned long long v = stile[i];
-      if (v == 0ull) continue;
+      if (v == 0ull) continue; /* entries outside the medium stay 0 */
       const uint32_t lx = i % A.tnx, r2 = i / A.tnx, ly = r2 % A.tny, lz = r2 / A.tny;
-      const MediumDev& tm = PERJOB ? J.med : A.med; /* the tile job's medium, not the lane's last one */
-      const uint32_t v_idx =
-          (uint32_t)A.tx0 + lx + (uint32_t)tm.nx * ((uint32_t)A.ty0 + ly + (uint32_t)tm.ny * ((uint32_t)A.tz0 + lz));
+      const uint32_t v_idx = tx0 + lx + (uint32_t)tm.nx * (ty0 + ly + (uint32_t)tm.ny * lz);
       atomic_add_u64(dst + v_idx, v);
     }
   }
@@ -719,7 +747,15 @@ extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int 
 }
 
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream) {
-  if (A->work < 0 || A->work > 2) return -1;
+  if (A->work < 0 || A->work > 2 || nblocks <= 0) return -1;
+  /* every launch reads the job table, the work counters and the multiplier
+     table; the other pointers as its work kind needs them */
+  if (!A->jobs || !A->counter || !A->a_tab || (A->n_mats && !A->mats) || A->njobs == 0 || A->njobs > MCFL_MAX_JOBS)
+    return -1;
+  if (A->work == MCFL_WORK_LANES && (!A->xs || !A->as)) return -1;
+  if (A->work == MCFL_WORK_VOXELS && (!A->vox || !A->vscale || A->per_job == 0)) return -1;
+  if (!(A->work == MCFL_WORK_JOBS && perjob) && !A->med.labels && A->med.n_layers == 0) return -1;
+  if (A->med.n_layers > 0 && !A->med.zb) return -1;
   const int layered = A->med.n_layers > 0u ? 1 : 0;
   const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered);
   if (!k || (layered && perjob)) return -1;
