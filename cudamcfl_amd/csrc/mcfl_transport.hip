@@ -358,7 +358,7 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
   }
   __syncthreads();
 
-  /* the medium: launch-uniform, or the current photon's job's (PERJOB) */
+  /* the medium: launch-uniform, or the wave's current job's (PERJOB) */
   MediumDev m = A.med;
   const uint32_t lane = __lane_id();
   /* XCD of this workgroup (HW_REG_XCC_ID[3:0]); selects the volume-tally copy */
@@ -387,6 +387,11 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
 
   /* wave-uniform chunk of the current job's photons: [pool_next, pool_next + pool_left) */
   uint32_t cur = swg[0], pool_left = 0;
+  /* PERJOB: the wave's photons share one medium (m stays wave-uniform), that
+     of job wmed; a wave moving on to a job in another medium lets its photons
+     finish first */
+  uint32_t wmed = cur;
+  if (PERJOB) m = A.jobs[cur].med;
   uint64_t pool_next = 0;
   bool pool_dead = false;
 
@@ -416,7 +421,9 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
         chunk = chunk < 64ull ? 64ull : (chunk > A.chunk_max ? (uint64_t)A.chunk_max : chunk);
         unsigned long long base = 0ull;
         if (lane == 0) base = atomicAdd(A.counter + cur, (unsigned long long)chunk);
-        base = __shfl(base, 0);
+        /* lane 0's result, as a wave-uniform (scalar) value */
+        base = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+               (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
         if (base >= cnt) { /* this job is exhausted: the next one, until the home job comes round again */
           cur = (cur + 1u == nslots) ? 0u : cur + 1u;
           pool_next = 0;
@@ -428,6 +435,11 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
         }
         pool_next = base;
         pool_left = (uint32_t)(base + chunk < cnt ? chunk : cnt - base);
+      }
+      if (PERJOB && cur != wmed) {
+        if (A.jobs[cur].med.labels != m.labels && __ballot(alive) != 0ull) break;
+        wmed = cur;
+        m = A.jobs[cur].med;
       }
       const uint32_t n = (uint32_t)__popcll(needmask);
       const uint32_t take = pool_left < n ? pool_left : n;
@@ -449,7 +461,6 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
           fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
           axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
           escale = J.exit_scale;
-          if (PERJOB) m = J.med;
           if (J.src_kind == MCFL_SRC_EMISSION) {
             /* importance-sampled emission voxel (Walker alias table), then an
                isotropic photon from its corner as in CUDAMCFLmain.cu:481-487 */
