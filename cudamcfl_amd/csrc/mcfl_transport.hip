@@ -385,8 +385,11 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
   uint32_t lane_item = 0;
   uint64_t lane_left = 0;
 
+  /* the tile's job and origin as scalars (no LDS read on voxel changes) */
+  const uint32_t tile_job = __builtin_amdgcn_readfirstlane(swg[1]);
+  const uint32_t tx0 = __builtin_amdgcn_readfirstlane(swg[2]), ty0 = __builtin_amdgcn_readfirstlane(swg[3]);
   /* wave-uniform chunk of the current job's photons: [pool_next, pool_next + pool_left) */
-  uint32_t cur = swg[0], pool_left = 0;
+  uint32_t cur = __builtin_amdgcn_readfirstlane(swg[0]), pool_left = 0;
   /* PERJOB: the wave's photons share one medium (m stays wave-uniform), that
      of job wmed; a wave moving on to a job in another medium lets its photons
      finish first */
@@ -599,8 +602,8 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
         fv = vi;
         facc = 0ull;
         ft = 0xFFFFFFFFu;
-        if (job == swg[1]) { /* inside the LDS tile of the workgroup's home job? */
-          const uint32_t lx = cl.ix - swg[2], ly = cl.iy - swg[3], lz = cl.iz;
+        if (job == tile_job) { /* inside the LDS tile of the workgroup's home job? */
+          const uint32_t lx = cl.ix - tx0, ly = cl.iy - ty0, lz = cl.iz;
           if (lx < A.tnx && ly < A.tny && lz < A.tnz) ft = lx + __umul24(A.tnx, ly + __umul24(A.tny, lz));
         }
       }
