@@ -468,6 +468,7 @@ def _bind_engine(L):
     L.mcfl_batch_run.argtypes = [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_double)]
     L.mcfl_batch_destroy.argtypes = [C.c_void_p]
     L.mcfl_batch_autotune.argtypes = [C.c_void_p, C.c_double, C.POINTER(Tune)]
+    L.mcfl_batch_configure.argtypes = [C.c_void_p, C.POINTER(Tune)]
     L.mcfl_batch_destroy.restype = None
     L.mcfl_run_batch.argtypes = [C.c_void_p, SP, C.c_int, C.POINTER(Job), C.c_int, C.POINTER(C.c_double)]
     L.mcfl_server_create.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]
@@ -484,7 +485,8 @@ def _bind_engine(L):
     L._engine = True
 
 
-EXPORTS += ["mcfl_batch_create", "mcfl_batch_run", "mcfl_batch_destroy", "mcfl_batch_autotune", "mcfl_run_batch",
+EXPORTS += ["mcfl_batch_create", "mcfl_batch_run", "mcfl_batch_destroy", "mcfl_batch_autotune", "mcfl_batch_configure",
+            "mcfl_run_batch",
             "mcfl_server_create",
             "mcfl_server_add_medium", "mcfl_server_submit", "mcfl_server_step", "mcfl_server_collect",
             "mcfl_server_stats", "mcfl_server_counters", "mcfl_server_destroy"]
@@ -522,7 +524,7 @@ class BatchJob:
 class Tune(C.Structure):
     _fields_ = [("tile_side", C.c_int32), ("tile_voxels", C.c_uint32), ("refill_min", C.c_uint32),
                 ("chunk_max", C.c_uint32), ("trial_ms", C.c_double), ("default_ms", C.c_double),
-                ("trials", C.c_int32)]
+                ("trials", C.c_int32), ("wg", C.c_uint32)]
 
 
 class Batch:
@@ -555,6 +557,12 @@ class Batch:
         t = Tune()
         check(lib().mcfl_batch_autotune(self.h, float(fraction), C.byref(t)), "mcfl_batch_autotune")
         return {n: getattr(t, n) for n, _ in Tune._fields_}
+
+    def configure(self, wg=0, tile_side=0, refill_min=4, chunk_max=1024):
+        """mcfl_batch_configure: set the launch settings (workgroup size, fhd tile side, refill
+        threshold, chunk cap) and re-capture the graph; results do not depend on them."""
+        t = Tune(tile_side=int(tile_side), refill_min=int(refill_min), chunk_max=int(chunk_max), wg=int(wg))
+        check(lib().mcfl_batch_configure(self.h, C.byref(t)), "mcfl_batch_configure")
 
     def close(self):
         if self.h:

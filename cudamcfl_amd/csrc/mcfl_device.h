@@ -28,6 +28,11 @@
 #ifndef MCFL_WAVES_PER_EU
 #define MCFL_WAVES_PER_EU 6 /* 80 VGPRs */
 #endif
+/* Workgroup sizes of the job kernels the batch autotuner chooses from: 512
+   (3 workgroups per CU at 6 waves/SIMD, the smallest tiles), 768 (the
+   default), 1024 (one workgroup per CU, 4 waves/SIMD, one 160 KB tile). */
+#define MCFL_WG_ALT1 512
+#define MCFL_WG_ALT2 1024
 #define MCFL_TALLY_COPIES 8      /* per-step volume tallies: one copy per XCD, summed after the launch */
 
 enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2 };
@@ -147,9 +152,13 @@ struct LaunchArgs {
   uint32_t n_waves;
   uint32_t chunk_max;
   uint32_t refill_min; /* a wave refills when this many of its lanes are idle (1..64) */
+  uint32_t wg;         /* threads per workgroup: 0 (MCFL_WG), or MCFL_WG_ALT1/2 for job launches */
 };
 
 /* Dynamic LDS of a launch: per-job totals, fhd tile, material table, layer bounds. */
+/* Threads per workgroup of a launch. */
+static inline uint32_t mcfl_wg_of(const LaunchArgs* A) { return A->wg ? A->wg : (uint32_t)MCFL_WG; }
+
 static inline uint32_t mcfl_lds_bytes(uint32_t nslots, uint32_t tile_n, uint32_t n_mats, uint32_t n_zb) {
   return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * 32u + n_zb * 8u;
 }

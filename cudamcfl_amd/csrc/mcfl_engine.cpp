@@ -399,11 +399,41 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
   plan_tiles(c, b->A, b->jd.data(), b->jd.size(), b->temporal, b->perjob);  // an LDS fhd tile per fhd job
   MCFL_STCHK(b->L.upload(b->jd, c->stream));
   b->nblocks = grid_blocks(c, b->A, b->temporal, b->perjob);
-  b->A.n_waves = (uint32_t)b->nblocks * (MCFL_WG / 64u);
+  b->A.n_waves = (uint32_t)b->nblocks * (mcfl_wg_of(&b->A) / 64u);
   MCFL_HIPCHK(hipStreamSynchronize(c->stream));
   MCFL_STCHK(capture(b.get()));
   *out = b.release();
   return MCFL_OK;
+}
+
+namespace {
+// Launch settings of the batch: workgroup size, fhd tile side (0: LDS budget
+// rule), refill threshold, chunk cap; re-places the tiles, uploads the job
+// table and re-captures the graph.
+mcfl_status apply_settings(mcfl_batch* b, uint32_t wg, int side, uint32_t refill, uint32_t chunk) {
+  mcfl_ctx* c = b->c;
+  LaunchArgs T = b->A;
+  T.wg = wg == MCFL_WG ? 0u : wg;
+  plan_tiles(c, T, b->jd.data(), b->jd.size(), b->temporal, b->perjob, side);
+  T.refill_min = refill;
+  T.chunk_max = chunk;
+  const int nblocks = grid_blocks(c, T, b->temporal, b->perjob);
+  T.n_waves = (uint32_t)nblocks * (mcfl_wg_of(&T) / 64u);
+  MCFL_STCHK(b->L.upload(b->jd, c->stream));
+  b->A = T;
+  b->nblocks = nblocks;
+  return capture(b);
+}
+}  // namespace
+
+extern "C" mcfl_status mcfl_batch_configure(mcfl_batch* b, const mcfl_tune* in) {
+  if (!b || !in) return MCFL_ERR_ARG;
+  const uint32_t wg = in->wg ? in->wg : (uint32_t)MCFL_WG;
+  if (wg != MCFL_WG && wg != MCFL_WG_ALT1 && wg != MCFL_WG_ALT2) return MCFL_ERR_ARG;
+  if (in->tile_side < 0 || in->refill_min < 1 || in->refill_min > 64 || in->chunk_max < 64 || in->chunk_max > (1u << 20))
+    return MCFL_ERR_ARG;
+  if (hipSetDevice(b->c->device) != hipSuccess) return MCFL_ERR_DEVICE;
+  return apply_settings(b, wg, in->tile_side, in->refill_min, in->chunk_max);
 }
 
 extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_tune* out) {
@@ -421,6 +451,10 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
     begin += n;
   }
   DevBuf d_trial;
+  struct Release {
+    DevBuf& b;
+    ~Release() { release(b); }
+  } rel{d_trial};
   MCFL_STCHK(ensure(d_trial, jd.size() * sizeof(JobDev)));
   int fhd_job = -1;
   for (size_t j = 0; j < b->jd.size(); ++j)
@@ -429,25 +463,20 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
       break;
     }
   struct Cand {
+    uint32_t wg;  // 0: MCFL_WG
     int side;
     uint32_t refill, chunk;
   };
-  std::vector<Cand> cands;
-  const int sides_fhd[] = {0, 16, 32};
-  const uint32_t refills[] = {4, 8}, chunks[] = {1024, 256};
-  for (int sd : sides_fhd) {
-    if (fhd_job < 0 && sd != 0) continue;  // the tile side only matters with an fhd tally
-    for (uint32_t r : refills)
-      for (uint32_t k : chunks) cands.push_back({sd, r, k});
-  }
   auto setup = [&](const Cand& cd, LaunchArgs& T, std::vector<JobDev>& tj, int* nblocks) {
     T = b->A;
+    T.wg = cd.wg;
     plan_tiles(c, T, tj.data(), tj.size(), b->temporal, b->perjob, cd.side);
     T.refill_min = cd.refill;
     T.chunk_max = cd.chunk;
     *nblocks = grid_blocks(c, T, b->temporal, b->perjob);
-    T.n_waves = (uint32_t)*nblocks * (MCFL_WG / 64u);
+    T.n_waves = (uint32_t)*nblocks * (mcfl_wg_of(&T) / 64u);
   };
+  int32_t trials = 0;
   auto trial = [&](const Cand& cd, double* ms) -> mcfl_status {
     LaunchArgs T;
     int nblocks = 0;
@@ -463,32 +492,44 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
     float f = 0.f;
     MCFL_HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
     *ms = f;
+    ++trials;
     return MCFL_OK;
   };
-  double warm = 0.0, best = 1e300, dflt = 0.0;
-  MCFL_STCHK(trial(cands[0], &warm));  // untimed warm-up of the default
-  size_t bi = 0;
-  for (size_t i = 0; i < cands.size(); ++i) {
+  double best = 1e300, dflt = 0.0;
+  Cand bc{0, 0, c->refill_min, c->chunk_max};
+  auto consider = [&](const Cand& cd) -> mcfl_status {  // best of two launches
     double t0 = 0.0, t1 = 0.0;
-    MCFL_STCHK(trial(cands[i], &t0));
-    MCFL_STCHK(trial(cands[i], &t1));
+    MCFL_STCHK(trial(cd, &t0));
+    MCFL_STCHK(trial(cd, &t1));
     const double t = std::min(t0, t1);
-    if (i == 0) dflt = t;
     if (t < best) {
       best = t;
-      bi = i;
+      bc = cd;
     }
+    return MCFL_OK;
+  };
+  double warm = 0.0;
+  MCFL_STCHK(trial(bc, &warm));  // untimed warm-up of the default
+  // stage 1: workgroup size, at the default tile rule, refill threshold and chunk cap
+  const Cand dc = bc;
+  MCFL_STCHK(consider(dc));
+  dflt = best;
+  for (uint32_t wg : {(uint32_t)MCFL_WG_ALT1, (uint32_t)MCFL_WG_ALT2}) MCFL_STCHK(consider({wg, dc.side, dc.refill, dc.chunk}));
+  // stage 2: tile side x refill threshold x chunk cap at that workgroup size
+  const uint32_t wg = bc.wg;
+  const int sides_fhd[] = {0, 16, 32};
+  const uint32_t refills[] = {4, 8}, chunks[] = {1024, 256};
+  for (int sd : sides_fhd) {
+    if (fhd_job < 0 && sd != 0) continue;  // the tile side only matters with an fhd tally
+    for (uint32_t r : refills)
+      for (uint32_t k : chunks) {
+        if (sd == dc.side && r == dc.refill && k == dc.chunk) continue;  // measured in stage 1
+        MCFL_STCHK(consider({wg, sd, r, k}));
+      }
   }
-  release(d_trial);
   // keep the fastest settings and capture the batch with them (trial tallies
   // are cleared by the graph itself)
-  LaunchArgs T;
-  int nblocks = 0;
-  setup(cands[bi], T, b->jd, &nblocks);
-  MCFL_STCHK(b->L.upload(b->jd, c->stream));
-  b->A = T;
-  b->nblocks = nblocks;
-  MCFL_STCHK(capture(b));
+  MCFL_STCHK(apply_settings(b, bc.wg, bc.side, bc.refill, bc.chunk));
   if (out) {
     out->tile_side = fhd_job >= 0 ? (int32_t)b->A.tnx : 0;
     out->tile_voxels = b->A.tile_n;
@@ -496,7 +537,8 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
     out->chunk_max = b->A.chunk_max;
     out->trial_ms = best;
     out->default_ms = dflt;
-    out->trials = (int32_t)(2 * cands.size() + 1);
+    out->trials = trials;
+    out->wg = mcfl_wg_of(&b->A);
   }
   return MCFL_OK;
 }
@@ -780,7 +822,7 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
       plan_tiles(c, A, jd.data(), jd.size(), 0, perjob ? 1 : 0);  // an LDS fhd tile per request tallying fhd
       MCFL_STCHK(E.L.upload(jd, E.stream));
       const int nblocks = grid_blocks(c, A, 0, perjob ? 1 : 0);
-      A.n_waves = (uint32_t)nblocks * (MCFL_WG / 64u);
+      A.n_waves = (uint32_t)nblocks * (mcfl_wg_of(&A) / 64u);
       MCFL_HIPCHK(hipMemsetAsync(A.counter, 0, 8 * jd.size(), E.stream));
       MCFL_HIPCHK(hipEventRecord(E.e0, E.stream));
       if (mcfl_transport_launch(&A, 0, perjob ? 1 : 0, nblocks, E.stream) != 0) return MCFL_ERR_DEVICE;

@@ -190,18 +190,27 @@ static uint32_t n_zb_of(const LaunchArgs& A) { return A.med.n_layers ? A.med.n_l
 int grid_blocks(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob) {
   const uint32_t nslots = A.work == MCFL_WORK_JOBS ? A.njobs : 1u;
   int bpc = 0;
-  if (mcfl_transport_occupancy(A.work, temporal, perjob, A.med.n_layers ? 1 : 0,
+  if (mcfl_transport_occupancy(A.work, temporal, perjob, A.med.n_layers ? 1 : 0, mcfl_wg_of(&A),
                                mcfl_lds_bytes(nslots, A.tile_n, A.n_mats, n_zb_of(A)), &bpc) != 0 ||
       bpc <= 0)
     bpc = 1;
   return c->n_cu * bpc;
 }
 
+// Register-limited resident workgroups per CU of A's kernel variant.
+static int bpc_of(const mcfl_ctx* c, const LaunchArgs& A, int temporal, int perjob) {
+  const int layered = A.med.n_layers ? 1 : 0;
+  if (mcfl_wg_of(&A) == MCFL_WG) return c->bpc[A.work][temporal ? 1 : 0][perjob ? 1 : 0][layered];
+  int bpc = 0;
+  if (mcfl_transport_occupancy(A.work, temporal, perjob, layered, mcfl_wg_of(&A), 0, &bpc) != 0) return 0;
+  return bpc;
+}
+
 void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, int temporal, int perjob, int side_req) {
   A.tile_n = 0;
   const uint32_t nslots = A.work == MCFL_WORK_JOBS ? A.njobs : 1u;
   const int layered = A.med.n_layers ? 1 : 0;
-  const int bpc = c->bpc[A.work][temporal ? 1 : 0][perjob ? 1 : 0][layered];
+  const int bpc = bpc_of(c, A, temporal, perjob);
   if (bpc <= 0) return;
   const size_t per_wg = c->lds_per_cu / (size_t)bpc;
   const size_t other = mcfl_lds_bytes(nslots, 0, A.n_mats, n_zb_of(A));
@@ -221,8 +230,8 @@ void choose_tile(const mcfl_ctx* c, LaunchArgs& A, const MediumDev& m, int tempo
   // keep the register-limited occupancy
   int got = 0;
   while (A.tile_n > 0 &&
-         (mcfl_transport_occupancy(A.work, temporal, perjob, layered, mcfl_lds_bytes(nslots, A.tile_n, A.n_mats, n_zb_of(A)),
-                                   &got) != 0 ||
+         (mcfl_transport_occupancy(A.work, temporal, perjob, layered, mcfl_wg_of(&A),
+                                   mcfl_lds_bytes(nslots, A.tile_n, A.n_mats, n_zb_of(A)), &got) != 0 ||
           got < bpc)) {
     if (A.tnz <= 1) {
       A.tile_n = 0;
@@ -261,7 +270,7 @@ void plan_tiles(const mcfl_ctx* c, LaunchArgs& A, JobDev* jd, size_t n, int temp
 mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob, double* ms,
                             const std::function<int()>& post) {
   const int nblocks = grid_blocks(c, A, temporal, perjob);
-  A.n_waves = (uint32_t)nblocks * (MCFL_WG / 64u);
+  A.n_waves = (uint32_t)nblocks * (mcfl_wg_of(&A) / 64u);
   if (A.chunk_max == 0) A.chunk_max = c->chunk_max;
   A.refill_min = c->refill_min;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -424,7 +433,7 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
       for (int pj = 0; pj < 2; ++pj)
         for (int ly = 0; ly < 2; ++ly) {
           if (ly && (pj || w == MCFL_WORK_VOXELS)) continue;  // no such variant
-          if (mcfl_transport_occupancy(w, t, pj, ly, 0, &c->bpc[w][t][pj][ly]) != 0) return fail(MCFL_ERR_DEVICE);
+          if (mcfl_transport_occupancy(w, t, pj, ly, 0, 0, &c->bpc[w][t][pj][ly]) != 0) return fail(MCFL_ERR_DEVICE);
         }
   if (prop.maxSharedMemoryPerMultiProcessor > 0) c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(MCFL_ERR_DEVICE);

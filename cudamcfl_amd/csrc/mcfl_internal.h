@@ -13,7 +13,7 @@
 #include "../../include/mcfl.h"
 #include "mcfl_device.h"
 
-extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int layered, uint32_t lds_bytes,
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int layered, uint32_t wg, uint32_t lds_bytes,
                                         int* blocks_per_cu);
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream);
 extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream);

@@ -321,8 +321,12 @@ __device__ __forceinline__ uint32_t job_at(const LaunchArgs& A, uint64_t g) {
 
 } /* namespace */
 
-template <int WORK, bool TEMPORAL, bool PERJOB, bool LAYERED>
-__global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WAVES_PER_EU))) void mcfl_transport_kernel(LaunchArgs A) {
+/* resident waves per SIMD the register allocation is sized for: a 1024-thread
+   workgroup (4 waves per SIMD) is alone on its CU */
+constexpr int waves_per_eu_of(int wg) { return wg > 768 ? 4 : MCFL_WAVES_PER_EU; }
+
+template <int WORK, bool TEMPORAL, bool PERJOB, bool LAYERED, int WGS>
+__global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_eu_of(WGS)))) void mcfl_transport_kernel(LaunchArgs A) {
   /* dynamic LDS: [per-job totals | fhd tile | materials | layer bounds] */
   extern __shared__ __align__(16) unsigned long long lds[];
 
@@ -332,10 +336,10 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
   unsigned long long* stile = lds + nslots * MCFL_NTOT;
   MatDev* smat = reinterpret_cast<MatDev*>(stile + A.tile_n);
   float* szb = reinterpret_cast<float*>(smat + A.n_mats);
-  for (uint32_t i = tid; i < A.n_mats; i += MCFL_WG) smat[i] = A.mats[i];
+  for (uint32_t i = tid; i < A.n_mats; i += WGS) smat[i] = A.mats[i];
   if (LAYERED)
-    for (uint32_t i = tid; i < 2u * (A.med.n_layers + 2u); i += MCFL_WG) szb[i] = A.med.zb[i];
-  for (uint32_t i = tid; i < nslots * MCFL_NTOT + A.tile_n; i += MCFL_WG) lds[i] = 0ull;
+    for (uint32_t i = tid; i < 2u * (A.med.n_layers + 2u); i += WGS) szb[i] = A.med.zb[i];
+  for (uint32_t i = tid; i < nslots * MCFL_NTOT + A.tile_n; i += WGS) lds[i] = 0ull;
   /* Work: each job has its own counter over its photons [0, count).  A
      workgroup starts on its home job (jobs get workgroups in proportion to
      their share of the photons), whose fhd it privatises in its LDS tile,
@@ -690,7 +694,7 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
   }
 
   __syncthreads();
-  for (uint32_t i = tid; i < nslots * MCFL_NTOT; i += MCFL_WG) {
+  for (uint32_t i = tid; i < nslots * MCFL_NTOT; i += WGS) {
     const unsigned long long v = stot[i];
     if (v != 0ull) atomic_add_u64(A.jobs[i / MCFL_NTOT].tot + (i % MCFL_NTOT), v);
   }
@@ -699,7 +703,7 @@ __global__ __launch_bounds__(MCFL_WG) __attribute__((amdgpu_waves_per_eu(MCFL_WA
     unsigned long long* dst = HJ.fhd + xcc * HJ.copy_stride;
     const MediumDev& tm = PERJOB ? HJ.med : A.med; /* the home job's medium, not the lane's last one */
     const uint32_t tx0 = swg[2], ty0 = swg[3];
-    for (uint32_t i = tid; i < A.tile_n; i += MCFL_WG) {
+    for (uint32_t i = tid; i < A.tile_n; i += WGS) {
       const unsigned long long v = stile[i];
       if (v == 0ull) continue; /* entries outside the medium stay 0 */
       const uint32_t lx = i % A.tnx, r2 = i / A.tnx, ly = r2 % A.tny, lz = r2 / A.tny;
@@ -727,13 +731,32 @@ __global__ void mcfl_reduce_copies_kernel(unsigned long long* p, uint64_t n, uin
 /* ------------------------------------------------------------------ host */
 
 namespace {
-template <int W, bool T, bool P, bool L>
+template <int W, bool T, bool P, bool L, int G = MCFL_WG>
 const void* kfn() {
-  return (const void*)mcfl_transport_kernel<W, T, P, L>;
+  return (const void*)mcfl_transport_kernel<W, T, P, L, G>;
 }
 /* perjob selects mixed-media launches (jobs only); layered the bulk-method-1
-   medium (jobs and reference lanes, one medium per launch) */
-const void* kernel_of(int work, int temporal, int perjob, int layered) {
+   medium (jobs and reference lanes, one medium per launch); wg the workgroup
+   size (MCFL_WG, or MCFL_WG_ALT1/2 for non-layered job launches) */
+const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t wg) {
+  if (wg != MCFL_WG) {
+    if (work != MCFL_WORK_JOBS || layered) return nullptr;
+    if (wg == MCFL_WG_ALT1) {
+      if (perjob)
+        return temporal ? kfn<MCFL_WORK_JOBS, true, true, false, MCFL_WG_ALT1>()
+                        : kfn<MCFL_WORK_JOBS, false, true, false, MCFL_WG_ALT1>();
+      return temporal ? kfn<MCFL_WORK_JOBS, true, false, false, MCFL_WG_ALT1>()
+                      : kfn<MCFL_WORK_JOBS, false, false, false, MCFL_WG_ALT1>();
+    }
+    if (wg == MCFL_WG_ALT2) {
+      if (perjob)
+        return temporal ? kfn<MCFL_WORK_JOBS, true, true, false, MCFL_WG_ALT2>()
+                        : kfn<MCFL_WORK_JOBS, false, true, false, MCFL_WG_ALT2>();
+      return temporal ? kfn<MCFL_WORK_JOBS, true, false, false, MCFL_WG_ALT2>()
+                      : kfn<MCFL_WORK_JOBS, false, false, false, MCFL_WG_ALT2>();
+    }
+    return nullptr;
+  }
   if (layered) {
     if (work == MCFL_WORK_JOBS)
       return temporal ? kfn<MCFL_WORK_JOBS, true, false, true>() : kfn<MCFL_WORK_JOBS, false, false, true>();
@@ -752,11 +775,11 @@ const void* kernel_of(int work, int temporal, int perjob, int layered) {
 }
 }  // namespace
 
-extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int layered, uint32_t lds_bytes,
+extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int layered, uint32_t wg, uint32_t lds_bytes,
                                         int* blocks_per_cu) {
-  const void* k = kernel_of(work, temporal, perjob, layered);
+  const void* k = kernel_of(work, temporal, perjob, layered, wg ? wg : MCFL_WG);
   if (!k) return -1;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, MCFL_WG, lds_bytes);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, (int)(wg ? wg : MCFL_WG), lds_bytes);
   return e == hipSuccess ? 0 : (int)e;
 }
 
@@ -771,11 +794,13 @@ extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perj
   if (!(A->work == MCFL_WORK_JOBS && perjob) && !A->med.labels && A->med.n_layers == 0) return -1;
   if (A->med.n_layers > 0 && !A->med.zb) return -1;
   const int layered = A->med.n_layers > 0u ? 1 : 0;
-  const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered);
+  const uint32_t wg = mcfl_wg_of(A);
+  const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered, wg);
   if (!k || (layered && perjob)) return -1;
+  if (A->n_waves != (uint32_t)nblocks * (wg / 64u)) return -1;
   void* args[] = {const_cast<LaunchArgs*>(A)};
   const uint32_t nslots = A->work == MCFL_WORK_JOBS ? A->njobs : 1u;
-  hipError_t e = hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(MCFL_WG), args,
+  hipError_t e = hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(wg), args,
                                  mcfl_lds_bytes(nslots, A->tile_n, A->n_mats, layered ? A->med.n_layers + 2u : 0u),
                                  stream);
   return e == hipSuccess ? 0 : (int)e;

@@ -267,13 +267,21 @@ typedef struct mcfl_tune {
   double trial_ms;      /* device time of the chosen settings on the scaled-down batch */
   double default_ms;    /* the same for the default settings */
   int32_t trials;       /* trial launches made */
+  uint32_t wg;          /* threads per workgroup (512, 768 or 1024) */
 } mcfl_tune;
 
-/* No counterpart in the reference.  Times candidate launch settings (fhd tile
- * side, refill threshold, chunk size) on a copy of the batch with every budget
- * scaled by `fraction` (at least 4096 photons per job), keeps the fastest and
+/* No counterpart in the reference.  Times candidate launch settings on a copy
+ * of the batch with every budget scaled by `fraction` (at least 4096 photons
+ * per job): first the workgroup size (512, 768, 1024 threads), then, at that
+ * size, fhd tile side x refill threshold x chunk size; keeps the fastest and
  * re-captures the graph with it.  Results do not depend on these settings. */
 mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_tune* out);
+/* No counterpart in the reference.  Applies launch settings (as returned by
+ * mcfl_batch_autotune, e.g. tuned once on another batch of the same shape):
+ * wg (0 or 768, 512, 1024), tile_side (0: the LDS budget rule), refill_min
+ * (1..64), chunk_max (64..2^20); the other fields are ignored.  Re-captures
+ * the graph.  Results do not depend on these settings. */
+mcfl_status mcfl_batch_configure(mcfl_batch* b, const mcfl_tune* in);
 /* create + run + destroy */
 mcfl_status mcfl_run_batch(mcfl_ctx* ctx, const mcfl_sim* const* media, int n_media, mcfl_job* jobs, int n_jobs,
                            double* kernel_ms);

@@ -170,9 +170,36 @@ def test_autotuned_batch_equals_default(ctx):
     before = [(j.Rd.copy(), j.fhd.copy(), j.time.copy(), dict(j.totals)) for j in jobs]
     t = b.autotune(0.25)
     assert t["trials"] > 1 and t["refill_min"] in (4, 8) and t["chunk_max"] in (256, 1024)
+    assert t["wg"] in (512, 768, 1024)
     assert t["tile_side"] > 0 and t["trial_ms"] <= t["default_ms"]
     b.run()
     for j, (rd, fhd, tim, tot) in zip(jobs, before):
         assert np.array_equal(j.Rd, rd) and np.array_equal(j.fhd, fhd) and np.array_equal(j.time, tim)
         assert j.totals == tot
+    b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_settings_do_not_change_results(ctx, mixed):
+    """Every workgroup size, tile side, refill threshold and chunk cap the
+    autotuner can choose gives the same integer tallies (one medium, and two
+    media in one launch)."""
+    sims = [medium(temporal=True)] + ([medium(16, 12, 8, 1, temporal=True)] if mixed else [])
+    srcs = [(0.0, 0.0, 0.0), (0.3, 0.1, 0.0), (-0.2, 0.2, 0.0)]
+    jobs = [M.BatchJob((i % len(sims)), 5000 + 700 * i, 70 + i, source=s,
+                       tallies=M.TALLY_FHD | M.TALLY_SQ | M.TALLY_TEMP) for i, s in enumerate(srcs)]
+    b = M.Batch(ctx, sims, jobs)
+    print(f"case: settings sweep, mixed={mixed}", flush=True)
+    b.run()
+    ref = [(j.Rd.copy(), j.Tt.copy(), j.fhd.copy(), j.time.copy(), j.Rd2.copy(), dict(j.totals)) for j in jobs]
+    for wg, side, refill, chunk in [(512, 0, 4, 1024), (1024, 0, 8, 256), (768, 16, 1, 64), (1024, 32, 64, 4096)]:
+        print(f"case: wg={wg} side={side} refill={refill} chunk={chunk}", flush=True)
+        b.configure(wg=wg, tile_side=side, refill_min=refill, chunk_max=chunk)
+        b.run()
+        for j, (rd, tt, fhd, tim, rd2, tot) in zip(jobs, ref):
+            assert np.array_equal(j.Rd, rd) and np.array_equal(j.Tt, tt) and np.array_equal(j.fhd, fhd)
+            assert np.array_equal(j.time, tim) and np.array_equal(j.Rd2, rd2) and j.totals == tot
+    with pytest.raises(M.McflError):
+        b.configure(wg=640)
     b.close()
