@@ -198,14 +198,10 @@ def bench_hetero(ctx):
             "speedup_wall": plain_wall / wall_b, "images_identical": same, "autotune": tune}
 
 
-def bench_serve(ctx, rate=3.0, duration=12.0, epoch=4_000_000):
-    """Poisson request stream: continuous batching vs a plain
-    one-launch-per-request FIFO loop on the same arrivals."""
+def _served(ctx, media, reqs, epoch):
+    """Continuous batching: requests are submitted at their arrival times and
+    the server steps whenever it has work.  Returns latencies, span, counters."""
     import cudamcfl_amd as M
-    from cudamcfl_amd.media import request_stream
-    media, reqs = request_stream(rate=rate, duration=duration)
-    work = sum(n * (2 if fl else 1) for (_, _, _, n, _, fl) in reqs)
-
     srv = M.Server(ctx, mem_budget=48 << 30, epoch_photons=epoch)
     mids = [srv.add_medium(m) for m in media]
     t0 = time.perf_counter()
@@ -227,74 +223,112 @@ def bench_serve(ctx, rate=3.0, duration=12.0, epoch=4_000_000):
         for rid in done:
             lat[rid] = tnow - arr[rid][0]
             srv.collect(rid, arr[rid][1])
-    span_s = time.perf_counter() - t0
+    span = time.perf_counter() - t0
     cnt = srv.counters()
     srv.close()
-    served = np.array(sorted(lat.values()))
-    print(f"served: {len(served)} requests, span {span_s:.1f} s", file=sys.stderr, flush=True)
+    return np.array(sorted(lat.values())), span, cnt
 
-    # plain loop: requests in arrival order, one launch each (+ one emission launch)
-    t0 = time.perf_counter()
-    plat = []
-    for (ta, m, src, n, seed, fl) in reqs:
-        now = time.perf_counter() - t0
-        if now < ta:
-            time.sleep(ta - now)
-        s = media[m]
-        s.s.number_of_photons = n
-        s.s.xi, s.s.yi, s.s.zi = src
-        s.s.src_kind = M.SOURCE_PENCIL
-        s.s.fhd_activated = 1 if fl else 0
-        r = ctx.run_excitation(s, seed=seed)
-        if fl:
-            Fx = r.fhd.astype(np.float64) / (4294967295.0 * r.totals["photons"])
-            job = M.BatchJob(0, n, seed, src_kind=M.SOURCE_EMISSION, Fx=Fx)
-            bb = M.Batch(ctx, [s], [job])
-            bb.run()
-            bb.close()
-        plat.append(time.perf_counter() - t0 - ta)
-    span_p = time.perf_counter() - t0
-    plain = np.array(sorted(plat))
-    print(f"plain: span {span_p:.1f} s", file=sys.stderr, flush=True)
 
-    # plain loop with the reference's launch geometry (32768 MWC lanes per simulation)
+def _plain(ctx, media, reqs, lanes=False):
+    """One launch per request (+ one emission launch) in arrival order; with
+    lanes, on the reference's 32768 per-lane MWC streams."""
+    import cudamcfl_amd as M
     t0 = time.perf_counter()
-    llat = []
+    lat = []
     for k, (ta, m, src, n, seed, fl) in enumerate(reqs):
         now = time.perf_counter() - t0
         if now < ta:
             time.sleep(ta - now)
         s = media[m]
-        s.s.number_of_photons = max(n, M.NUM_THREADS)
+        s.s.number_of_photons = max(n, M.NUM_THREADS) if lanes else n
         s.s.xi, s.s.yi, s.s.zi = src
         s.s.src_kind = M.SOURCE_PENCIL
         s.s.fhd_activated = 1 if fl else 0
-        x, a = M.init_rng(M.NUM_THREADS, seed=seed)
-        r = ctx.run_excitation(s, lanes=(x, a))
+        if lanes:
+            x, a = M.init_rng(M.NUM_THREADS, seed=seed)
+            r = ctx.run_excitation(s, lanes=(x, a))
+        else:
+            r = ctx.run_excitation(s, seed=seed)
         if fl:
             Fx = r.fhd.astype(np.float64) / (4294967295.0 * r.totals["photons"])
             job = M.BatchJob(0, n, seed, src_kind=M.SOURCE_EMISSION, Fx=Fx)
             bb = M.Batch(ctx, [s], [job])
             bb.run()
             bb.close()
-        llat.append(time.perf_counter() - t0 - ta)
-        if k % 8 == 7:
+        lat.append(time.perf_counter() - t0 - ta)
+        if lanes and k % 8 == 7:
             print(f"plain, reference lanes: {k + 1} requests", file=sys.stderr, flush=True)
-    span_l = time.perf_counter() - t0
-    lanes = np.array(sorted(llat))
-    print(f"plain, reference lanes: span {span_l:.1f} s", file=sys.stderr, flush=True)
-    return {"workload": f"Poisson arrivals {rate}/s for {duration}s ({len(reqs)} requests), 8 media up to "
-                        f"128x128x64, scan sources, 50% with fluorescence, budgets log-uniform [1e5,1e8]",
-            "photons": int(work), "epoch_photons": epoch,
-            "served": {"p50_s": float(np.percentile(served, 50)), "p99_s": float(np.percentile(served, 99)),
-                       "photons_per_s": work / span_s, "span_s": span_s, "epochs": cnt["epochs"],
-                       "device_photons_per_s": cnt["photons"] / max(cnt["kernel_ms"], 1e-9) * 1e3,
-                       "kernel_s": cnt["kernel_ms"] / 1e3, "host_s": cnt["host_ms"] / 1e3},
-            "plain": {"p50_s": float(np.percentile(plain, 50)), "p99_s": float(np.percentile(plain, 99)),
-                      "photons_per_s": work / span_p, "span_s": span_p},
-            "plain_reference_lanes": {"p50_s": float(np.percentile(lanes, 50)),
-                                      "p99_s": float(np.percentile(lanes, 99)),
-                                      "photons_per_s": work / span_l, "span_s": span_l}}
+    return np.array(sorted(lat)), time.perf_counter() - t0
+
+
+def bench_serve(ctx, rate=3.0, high_rate=12.0, duration=12.0, epoch=250_000):
+    """Poisson request streams: continuous batching vs a plain
+    one-launch-per-request FIFO loop on the same arrivals, at a light load
+    (also against the reference's launch geometry) and at a high load."""
+    from cudamcfl_amd.media import request_stream
+
+    def lat_stats(lat, work, span):
+        return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                "photons_per_s": work / span, "span_s": span}
+
+    def one(r, with_lanes):
+        media, reqs = request_stream(rate=r, duration=duration)
+        work = sum(n * (2 if fl else 1) for (_, _, _, n, _, fl) in reqs)
+        served, span_s, cnt = _served(ctx, media, reqs, epoch)
+        print(f"served at {r}/s: {len(served)} requests, span {span_s:.1f} s", file=sys.stderr, flush=True)
+        plain, span_p = _plain(ctx, media, reqs)
+        print(f"plain at {r}/s: span {span_p:.1f} s", file=sys.stderr, flush=True)
+        out = {"workload": f"Poisson arrivals {r}/s for {duration}s ({len(reqs)} requests), 8 media up to "
+                           f"128x128x64, scan sources, 50% with fluorescence, budgets log-uniform [1e5,1e8]",
+               "photons": int(work), "epoch_photons": epoch,
+               "served": dict(lat_stats(served, work, span_s), epochs=cnt["epochs"],
+                              device_photons_per_s=cnt["photons"] / max(cnt["kernel_ms"], 1e-9) * 1e3,
+                              kernel_s=cnt["kernel_ms"] / 1e3, host_s=cnt["host_ms"] / 1e3),
+               "plain": lat_stats(plain, work, span_p)}
+        if with_lanes:
+            lanes, span_l = _plain(ctx, media, reqs, lanes=True)
+            print(f"plain, reference lanes: span {span_l:.1f} s", file=sys.stderr, flush=True)
+            out["plain_reference_lanes"] = lat_stats(lanes, work, span_l)
+        return out
+
+    out = one(rate, True)
+    out["high_load"] = one(high_rate, False)
+    return out
+
+
+def bench_sample2(ctx, frac=1.0):
+    """The reference's own sample2.mci end to end: excitation (temporal
+    detectors on) and the fluorescence stage over every emitting voxel,
+    photon counts scaled by frac."""
+    import numpy as np
+    import cudamcfl_amd as M
+    d = os.path.join(ROOT, "tests", "golden", "ref_inputs")
+    cwd = os.getcwd()
+    os.chdir(d)  # the bulk file is named relative to the working directory
+    try:
+        sim = M.Simulation.from_mci("sample2.mci")
+    finally:
+        os.chdir(cwd)
+    s = sim.s
+    n_ex = max(1, int(s.number_of_photons * frac))
+    n_v = max(1, int(s.number_of_photons_per_voxel * frac))
+    s.number_of_photons = n_ex
+    t0 = time.perf_counter()
+    ex = ctx.run_excitation(sim, seed=1)
+    t1 = time.perf_counter()
+    Fx = ex.fhd.astype(np.float64) / (4294967295.0 * ex.totals["photons"])
+    fl = ctx.run_fluorescence(sim, Fx, n_v, seed=2)
+    t2 = time.perf_counter()
+    return {"workload": f"sample2.mci ({s.nx}x{s.ny}x{s.nz} voxels from 3dbulk-sample, temporal detectors), "
+                        f"photon counts x{frac:g}",
+            "excitation": {"photons": n_ex, "device_ms": ex.kernel_ms, "wall_s": t1 - t0,
+                           "photons_per_s": n_ex / ex.kernel_ms * 1e3,
+                           "steps_per_photon": ex.totals["steps"] / n_ex},
+            "fluorescence": {"voxels": fl["voxels"], "photons_per_voxel": n_v, "photons": fl["photons"],
+                             "device_ms": fl["kernel_ms"], "wall_s": t2 - t1,
+                             "photons_per_s": fl["photons"] / fl["kernel_ms"] * 1e3,
+                             "steps_per_photon": fl["totals"]["steps"] / max(1, fl["photons"])},
+            "total_wall_s": t2 - t0}
 
 
 def main():
@@ -308,6 +342,8 @@ def main():
     ap.add_argument("--scan", action="store_true", help="also run the 64-source batched scan")
     ap.add_argument("--hetero", action="store_true", help="also run the 32-medium heterogeneous batch")
     ap.add_argument("--serve", action="store_true", help="also run the served request stream")
+    ap.add_argument("--sample2", type=float, default=0.0,
+                    help="also run the reference's sample2.mci end to end, photon counts scaled by this factor")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -376,6 +412,8 @@ def main():
         out["batched_hetero"] = bench_hetero(ctx)
     if args.serve:
         out["serving"] = bench_serve(ctx)
+    if args.sample2 > 0:
+        out["sample2_end_to_end"] = bench_sample2(ctx, args.sample2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

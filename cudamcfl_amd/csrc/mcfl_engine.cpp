@@ -97,6 +97,82 @@ void build_alias(const std::vector<double>& w, std::vector<uint32_t>& prob, std:
 }
 
 // Device tallies of one job (or one phase of a served request).
+// Device buffers recycled between served requests.  hipFree waits for the
+// whole device (both epochs in flight), so retired requests hand their
+// buffers to the pool and admitted ones take them back (best fit, at most
+// twice the size asked for); only what exceeds `limit` bytes is freed.
+struct BufPool {
+  std::multimap<size_t, void*> free_;
+  size_t held = 0, limit = 0;
+  ~BufPool() {
+    for (auto& kv : free_) (void)hipFree(kv.second);
+  }
+  mcfl_status get(DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return MCFL_OK;
+    put(b);
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first / 2 <= bytes) {
+      b.p = it->second;
+      b.cap = it->first;
+      held -= it->first;
+      free_.erase(it);
+      return MCFL_OK;
+    }
+    return ensure(b, bytes);
+  }
+  void put(DevBuf& b) {
+    if (!b.p) return;
+    free_.emplace(b.cap, b.p);
+    held += b.cap;
+    b.p = nullptr;
+    b.cap = 0;
+    while (held > limit && !free_.empty()) {
+      auto it = std::prev(free_.end());
+      held -= it->first;
+      (void)hipFree(it->second);
+      free_.erase(it);
+    }
+  }
+};
+
+// Pinned host staging buffers recycled between served requests (the same
+// reason: freeing pinned memory is not free either); at most `limit` bytes
+// are kept.
+struct HostPool {
+  std::multimap<size_t, uint8_t*> free_;
+  size_t held = 0, limit = size_t(1) << 30;
+  ~HostPool() {
+    for (auto& kv : free_) (void)hipHostFree(kv.second);
+  }
+  mcfl_status get(uint8_t** p, size_t* cap, size_t bytes) {
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first / 2 <= bytes) {
+      *p = it->second;
+      *cap = it->first;
+      held -= it->first;
+      free_.erase(it);
+      return MCFL_OK;
+    }
+    if (hipHostMalloc((void**)p, bytes, hipHostMallocDefault) != hipSuccess) return MCFL_ERR_NOMEM;
+    *cap = bytes;
+    return MCFL_OK;
+  }
+  void put(uint8_t** p, size_t* cap) {
+    if (!*p) return;
+    free_.emplace(*cap, *p);
+    held += *cap;
+    *p = nullptr;
+    *cap = 0;
+    while (held > limit && !free_.empty()) {
+      auto it = std::prev(free_.end());
+      held -= it->first;
+      (void)hipHostFree(it->second);
+      free_.erase(it);
+    }
+  }
+};
+
 struct Tallies {
   DevBuf Rd, Tt, fhd, axyz, Rd2, Tt2, time, tot, al_prob, al_alias, al_vox;
   uint32_t flags = 0;
@@ -104,37 +180,44 @@ struct Tallies {
   uint32_t al_n = 0;
   double fl_unit = 0.0;
   size_t bytes = 0;
+  BufPool* pool = nullptr; /* served requests: buffers come from and go back to the server's pool */
+  mcfl_status get(DevBuf& b, size_t n) { return pool ? pool->get(b, n) : ensure(b, n); }
   ~Tallies() {
-    for (DevBuf* b : {&Rd, &Tt, &fhd, &axyz, &Rd2, &Tt2, &time, &tot, &al_prob, &al_alias, &al_vox}) release(*b);
+    for (DevBuf* b : {&Rd, &Tt, &fhd, &axyz, &Rd2, &Tt2, &time, &tot, &al_prob, &al_alias, &al_vox}) {
+      if (pool)
+        pool->put(*b);
+      else
+        release(*b);
+    }
   }
 };
 
 mcfl_status alloc_tallies(const Medium& m, uint32_t tallies, bool emission, uint32_t copies, Tallies* t) {
   const size_t npix = m.npix * 8, nvox = m.nvox * 8 * copies;
   t->copies = copies;
-  MCFL_STCHK(ensure(t->Rd, npix));
-  MCFL_STCHK(ensure(t->Tt, npix));
-  MCFL_STCHK(ensure(t->tot, MCFL_NTOT * 8));
+  MCFL_STCHK(t->get(t->Rd, npix));
+  MCFL_STCHK(t->get(t->Tt, npix));
+  MCFL_STCHK(t->get(t->tot, MCFL_NTOT * 8));
   t->bytes = 2 * npix + MCFL_NTOT * 8;
   t->flags = 0;
   if ((tallies & MCFL_TALLY_FHD) && !emission) {
-    MCFL_STCHK(ensure(t->fhd, nvox));
+    MCFL_STCHK(t->get(t->fhd, nvox));
     t->flags |= MCFL_F_FHD;
     t->bytes += nvox;
   }
   if (tallies & MCFL_TALLY_AXYZ) {
-    MCFL_STCHK(ensure(t->axyz, nvox));
+    MCFL_STCHK(t->get(t->axyz, nvox));
     t->flags |= MCFL_F_AXYZ;
     t->bytes += nvox;
   }
   if (tallies & MCFL_TALLY_SQ) {
-    MCFL_STCHK(ensure(t->Rd2, npix));
-    MCFL_STCHK(ensure(t->Tt2, npix));
+    MCFL_STCHK(t->get(t->Rd2, npix));
+    MCFL_STCHK(t->get(t->Tt2, npix));
     t->flags |= MCFL_F_SQ;
     t->bytes += 2 * npix;
   }
   if ((tallies & MCFL_TALLY_TEMP) && m.sim.do_temp_sim == 1 && !emission) {
-    MCFL_STCHK(ensure(t->time, ntime_of(&m.sim) * 8));
+    MCFL_STCHK(t->get(t->time, ntime_of(&m.sim) * 8));
     t->flags |= MCFL_F_TEMP;
     t->bytes += ntime_of(&m.sim) * 8;
   }
@@ -162,7 +245,11 @@ int reduce_tallies(const Medium& m, Tallies* t, hipStream_t st) {
 // Emission sources of a medium given its normalised fhd: weights
 // eY (1 - albedof) Fx over voxels whose material has albedof < 1
 // (CUDAMCFLmain.cu:550-556); returns the sum.
-mcfl_status build_emission(const Medium& m, const double* Fx, uint64_t photons, Tallies* t) {
+// stage (optional): pinned host memory of at least 12 bytes per voxel; the
+// tables are then uploaded asynchronously on st from it (stage must stay
+// untouched until st has passed the copies).
+mcfl_status build_emission(const Medium& m, const double* Fx, uint64_t photons, Tallies* t, hipStream_t st = nullptr,
+                           uint8_t* stage = nullptr) {
   std::vector<double> w;
   std::vector<uint32_t> vox;
   w.reserve(m.nvox);
@@ -183,12 +270,21 @@ mcfl_status build_emission(const Medium& m, const double* Fx, uint64_t photons, 
   std::vector<uint32_t> prob, alias;
   build_alias(w, prob, alias);
   const size_t n = vox.size() * 4;
-  MCFL_STCHK(ensure(t->al_prob, n));
-  MCFL_STCHK(ensure(t->al_alias, n));
-  MCFL_STCHK(ensure(t->al_vox, n));
-  MCFL_HIPCHK(hipMemcpy(t->al_prob.p, prob.data(), n, hipMemcpyHostToDevice));
-  MCFL_HIPCHK(hipMemcpy(t->al_alias.p, alias.data(), n, hipMemcpyHostToDevice));
-  MCFL_HIPCHK(hipMemcpy(t->al_vox.p, vox.data(), n, hipMemcpyHostToDevice));
+  MCFL_STCHK(t->get(t->al_prob, n));
+  MCFL_STCHK(t->get(t->al_alias, n));
+  MCFL_STCHK(t->get(t->al_vox, n));
+  if (stage) {
+    memcpy(stage, prob.data(), n);
+    memcpy(stage + n, alias.data(), n);
+    memcpy(stage + 2 * n, vox.data(), n);
+    MCFL_HIPCHK(hipMemcpyAsync(t->al_prob.p, stage, n, hipMemcpyHostToDevice, st));
+    MCFL_HIPCHK(hipMemcpyAsync(t->al_alias.p, stage + n, n, hipMemcpyHostToDevice, st));
+    MCFL_HIPCHK(hipMemcpyAsync(t->al_vox.p, stage + 2 * n, n, hipMemcpyHostToDevice, st));
+  } else {
+    MCFL_HIPCHK(hipMemcpy(t->al_prob.p, prob.data(), n, hipMemcpyHostToDevice));
+    MCFL_HIPCHK(hipMemcpy(t->al_alias.p, alias.data(), n, hipMemcpyHostToDevice));
+    MCFL_HIPCHK(hipMemcpy(t->al_vox.p, vox.data(), n, hipMemcpyHostToDevice));
+  }
   t->bytes += 3 * n;
   return MCFL_OK;
 }
@@ -609,6 +705,21 @@ struct Request {
   std::vector<uint64_t> Rd, Tt, FlR, FlT;
   uint64_t tot[MCFL_NTOT] = {}, fl_tot[MCFL_NTOT] = {};
   double fl_unit = 0.0;
+  // A finished phase's tallies are copied out asynchronously (a synchronous
+  // copy or reduction would wait for the epoch kernel occupying the GPU):
+  // pending until `ev` has passed; pinned staging [images | totals | fhd or
+  // emission tables].
+  bool pending = false;
+  hipEvent_t ev = nullptr;
+  uint8_t* stage = nullptr;
+  size_t stage_bytes = 0;
+  ~Request() {
+    if (ev) (void)hipEventDestroy(ev);
+    if (stage) (void)hipHostFree(stage);  // normally handed back to the server's pool at retirement
+  }
+  size_t img_bytes() const { return m->npix * 8; }
+  uint8_t* stage_tot() const { return stage + 2 * img_bytes(); }
+  uint8_t* stage_vol() const { return stage_tot() + MCFL_NTOT * 8; }
 };
 
 size_t footprint_of(const Medium& m, const mcfl_request& r, uint32_t copies) {
@@ -633,7 +744,10 @@ struct Epoch {
 struct mcfl_server {
   mcfl_ctx* c = nullptr;
   uint64_t budget = 0, epoch_photons = 0, in_use = 0;
-  uint64_t epoch_target = 10000000;  // photons per epoch to aim for (env MCFL_EPOCH_TARGET)
+  BufPool pool;    // declared before the requests, so it outlives their tallies
+  HostPool hpool;  // pinned staging of the copy-outs
+  uint64_t epoch_target = 20000000;  // photons per epoch to aim for (env MCFL_EPOCH_TARGET)
+  int reserve = 8;                   // workgroup slots an epoch leaves free (env MCFL_SERVER_RESERVE)
   std::vector<std::unique_ptr<Medium>> media;
   std::deque<std::unique_ptr<Request>> queue;
   std::vector<std::unique_ptr<Request>> active;
@@ -647,7 +761,16 @@ struct mcfl_server {
   // cumulative counters
   uint64_t photons = 0, epochs = 0;
   double kernel_ms = 0.0, host_ms = 0.0;
+  // host time by part (ms): admission, epoch assembly, launch calls, phase
+  // changes to emission, retirements; printed at destruction when the
+  // environment sets MCFL_SERVER_PROFILE
+  double t_admit = 0.0, t_build = 0.0, t_launch = 0.0, t_emission = 0.0, t_retire = 0.0;
   ~mcfl_server() {
+    if (getenv("MCFL_SERVER_PROFILE"))
+      fprintf(stderr,
+              "mcfl_server: %llu epochs, host %.1f ms (admit %.1f, assemble %.1f, launch %.1f, to emission %.1f, "
+              "retire %.1f)\n",
+              (unsigned long long)epochs, host_ms, t_admit, t_build, t_launch, t_emission, t_retire);
     for (Epoch& e : ep) {
       if (e.stream) (void)hipStreamSynchronize(e.stream);
       if (e.e0) (void)hipEventDestroy(e.e0);
@@ -663,8 +786,10 @@ extern "C" mcfl_status mcfl_server_create(mcfl_ctx* c, uint64_t budget, uint64_t
   if (!s) return MCFL_ERR_NOMEM;
   s->c = c;
   s->budget = budget;
+  s->pool.limit = budget;  // recycled buffers kept: at most the admission budget again
   s->epoch_photons = epoch_photons;
   if (const char* e = getenv("MCFL_EPOCH_TARGET")) s->epoch_target = strtoull(e, nullptr, 10);
+  if (const char* e = getenv("MCFL_SERVER_RESERVE")) s->reserve = std::max(0, atoi(e));
   if (hipSetDevice(c->device) != hipSuccess) {
     delete s;
     return MCFL_ERR_DEVICE;
@@ -714,12 +839,14 @@ mcfl_status admit(mcfl_server* s) {
     Request& q = *s->queue.front();
     if (s->in_use + q.footprint > s->budget) break;  // admission by memory footprint, FIFO
     q.ex.reset(new Tallies());
+    q.ex->pool = &s->pool;
     // cleared on the stream of the next launch, the first one that can carry the request
     const hipStream_t st = s->ep[s->next].stream;
     MCFL_STCHK(alloc_tallies(*q.m, q.r.fluorescence ? (uint32_t)MCFL_TALLY_FHD : 0u, false, s->c->copies, q.ex.get()));
     MCFL_STCHK(clear_tallies(*q.m, q.ex.get(), st));
     if (q.r.fluorescence) {
       q.em.reset(new Tallies());
+      q.em->pool = &s->pool;
       MCFL_STCHK(alloc_tallies(*q.m, 0u, true, s->c->copies, q.em.get()));
       MCFL_STCHK(clear_tallies(*q.m, q.em.get(), st));
     }
@@ -731,37 +858,63 @@ mcfl_status admit(mcfl_server* s) {
   return MCFL_OK;
 }
 
-// Phase finished: start the emission pass or retire the request.
-mcfl_status advance(mcfl_server* s, Request& q, hipStream_t st) {
+// A phase's photons are all done: queue the copy-out of its tallies (after
+// summing the fhd copies when the emission pass follows) on the stream of the
+// epoch that finished it, and mark the request pending.
+mcfl_status start_copy_out(HostPool& pool, Request& q, hipStream_t st) {
   const Medium& m = *q.m;
+  const bool to_emission = q.phase == 1 && q.r.fluorescence;
+  Tallies& t = q.phase == 1 ? *q.ex : *q.em;
+  const size_t need = 2 * q.img_bytes() + MCFL_NTOT * 8 + (q.r.fluorescence ? std::max(m.nvox * 8, m.nvox * 12) : 0);
+  if (q.stage_bytes < need) {
+    pool.put(&q.stage, &q.stage_bytes);
+    MCFL_STCHK(pool.get(&q.stage, &q.stage_bytes, need));
+  }
+  if (!q.ev) MCFL_HIPCHK(hipEventCreateWithFlags(&q.ev, hipEventDisableTiming));
+  if (to_emission) {
+    if (reduce_tallies(m, &t, st) != 0) return MCFL_ERR_DEVICE;
+    MCFL_HIPCHK(hipMemcpyAsync(q.stage_vol(), t.fhd.p, m.nvox * 8, hipMemcpyDeviceToHost, st));
+  }
+  MCFL_HIPCHK(hipMemcpyAsync(q.stage, t.Rd.p, q.img_bytes(), hipMemcpyDeviceToHost, st));
+  MCFL_HIPCHK(hipMemcpyAsync(q.stage + q.img_bytes(), t.Tt.p, q.img_bytes(), hipMemcpyDeviceToHost, st));
+  MCFL_HIPCHK(hipMemcpyAsync(q.stage_tot(), t.tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost, st));
+  MCFL_HIPCHK(hipEventRecord(q.ev, st));
+  q.pending = true;
+  return MCFL_OK;
+}
+
+// The copy-out has landed: keep the phase's results, then start the emission
+// pass (its sampling tables uploaded on `up`, the stream of the next launch)
+// or retire the request.
+mcfl_status finish_copy_out(mcfl_server* s, Request& q, hipStream_t up) {
+  const Medium& m = *q.m;
+  q.pending = false;
+  const size_t npix = m.npix;
   if (q.phase == 1) {
-    q.Rd.resize(m.npix);
-    q.Tt.resize(m.npix);
-    MCFL_STCHK(download(m, *q.ex, q.Rd.data(), q.Tt.data(), nullptr, nullptr, nullptr, nullptr, nullptr, q.tot));
+    q.Rd.assign((const uint64_t*)q.stage, (const uint64_t*)q.stage + npix);
+    q.Tt.assign((const uint64_t*)(q.stage + q.img_bytes()), (const uint64_t*)(q.stage + q.img_bytes()) + npix);
+    memcpy(q.tot, q.stage_tot(), sizeof q.tot);
     if (q.r.fluorescence) {
-      if (reduce_tallies(m, q.ex.get(), st) != 0) return MCFL_ERR_DEVICE;
-      MCFL_HIPCHK(hipStreamSynchronize(st));
-      std::vector<uint64_t> fhd(m.nvox);
-      MCFL_HIPCHK(hipMemcpy(fhd.data(), q.ex->fhd.p, m.nvox * 8, hipMemcpyDeviceToHost));
+      const uint64_t* fhd = (const uint64_t*)q.stage_vol();
       std::vector<double> Fx(m.nvox);
       const double norm = 4294967295.0 * (double)q.tot[MCFL_TOT_PHOTONS];  // CUDAMCFLmain.cu:181-183
       for (size_t v = 0; v < m.nvox; ++v) Fx[v] = (double)fhd[v] / norm;
-      MCFL_STCHK(build_emission(m, Fx.data(), q.r.photons, q.em.get()));
+      MCFL_STCHK(build_emission(m, Fx.data(), q.r.photons, q.em.get(), up, q.stage_vol()));
       q.fl_unit = q.em->fl_unit;
       q.phase = 2;
       q.done = 0;
       q.finished = 0;
       if (q.em->al_n > 0) return MCFL_OK;
     }
-  }
-  if (q.phase == 2) {
-    q.FlR.resize(m.npix);
-    q.FlT.resize(m.npix);
-    MCFL_STCHK(download(m, *q.em, q.FlR.data(), q.FlT.data(), nullptr, nullptr, nullptr, nullptr, nullptr, q.fl_tot));
+  } else {
+    q.FlR.assign((const uint64_t*)q.stage, (const uint64_t*)q.stage + npix);
+    q.FlT.assign((const uint64_t*)(q.stage + q.img_bytes()), (const uint64_t*)(q.stage + q.img_bytes()) + npix);
+    memcpy(q.fl_tot, q.stage_tot(), sizeof q.fl_tot);
   }
   q.phase = 3;
   q.ex.reset();
   q.em.reset();
+  s->hpool.put(&q.stage, &q.stage_bytes);
   s->in_use -= q.footprint;
   return MCFL_OK;
 }
@@ -777,21 +930,56 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
   if (kernel_ms) *kernel_ms = 0.0;
   const auto t_host0 = std::chrono::steady_clock::now();
   double t_wait = 0.0;
+  auto since = [](std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+  };
   MCFL_STCHK(admit(s));
+  s->t_admit += since(t_host0);
+  // 0. Copy-outs that have landed move their requests on (emission or retirement).
+  auto finish_landed = [&](bool wait) -> mcfl_status {
+    for (std::unique_ptr<Request>& q : s->active) {
+      if (!q->pending) continue;
+      if (wait) MCFL_HIPCHK(hipEventSynchronize(q->ev));
+      const hipError_t e = hipEventQuery(q->ev);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) return MCFL_ERR_DEVICE;
+      const auto ta = std::chrono::steady_clock::now();
+      const bool to_emission = q->phase == 1 && q->r.fluorescence;
+      MCFL_STCHK(finish_copy_out(s, *q, s->ep[s->next].stream));
+      (to_emission ? s->t_emission : s->t_retire) += since(ta);
+    }
+    return MCFL_OK;
+  };
+  MCFL_STCHK(finish_landed(false));
+  const auto t_build0 = std::chrono::steady_clock::now();
 
-  // 1. Launch the next epoch if its slot is free.  Every active request with
-  // photons left in its phase advances by an equal quantum: the epoch target
-  // (photons of about 25 ms of device time) split over them, at least
-  // epoch_photons.  A launch is therefore large enough to fill the GPU, a lone
-  // large request finishes in a few epochs, and each request gets an equal
-  // share of the device.
+  // 1. Launch the next epoch if its slot is free.  An epoch carries about
+  // epoch_target photons (about 45 ms of device time): every active request
+  // with photons left in its phase gets at least epoch_photons of them, and
+  // the rest of the target goes to the requests with the least work left
+  // first (shortest remaining work first), so small requests finish in the
+  // next epoch while large ones keep advancing.
   bool launched = false;
   Epoch& E = s->ep[s->next];
-  size_t n_open = 0;
+  std::vector<Request*> open;
   for (const std::unique_ptr<Request>& qp : s->active)
-    if ((qp->phase == 1 || qp->phase == 2) && qp->done < qp->r.photons) ++n_open;
-  // per-request share of the epoch target, never below epoch_photons
-  const uint64_t quantum = std::max<uint64_t>(s->epoch_photons, n_open ? s->epoch_target / n_open : 0);
+    if ((qp->phase == 1 || qp->phase == 2) && qp->done < qp->r.photons) open.push_back(qp.get());
+  auto left_of = [](const Request* q) {  // photons left in the whole request
+    return (q->r.photons - q->done) + (q->phase == 1 && q->r.fluorescence ? q->r.photons : 0);
+  };
+  std::stable_sort(open.begin(), open.end(), [&](const Request* a, const Request* b) { return left_of(a) < left_of(b); });
+  std::map<int64_t, uint64_t> share;
+  {
+    uint64_t floors = 0;
+    for (const Request* q : open) floors += std::min<uint64_t>(q->r.photons - q->done, s->epoch_photons);
+    uint64_t extra = s->epoch_target > floors ? s->epoch_target - floors : 0;
+    for (const Request* q : open) {
+      const uint64_t rem = q->r.photons - q->done, fl = std::min<uint64_t>(rem, s->epoch_photons);
+      const uint64_t add = std::min<uint64_t>(rem - fl, extra);
+      extra -= add;
+      share[q->id] = fl + add;
+    }
+  }
   if (!E.busy) {
     std::vector<JobDev> jd;
     MatTable mt;
@@ -802,7 +990,7 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
     for (std::unique_ptr<Request>& qp : s->active) {
       Request& q = *qp;
       if ((q.phase != 1 && q.phase != 2) || q.done >= q.r.photons) continue;
-      const uint64_t take = std::min(q.r.photons - q.done, quantum);
+      const uint64_t take = share[q.id];
       if (!uni) uni = q.m;
       if (q.m != uni) perjob = true;
       const bool em = q.phase == 2;
@@ -819,14 +1007,19 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
     if (!jd.empty()) {
       LaunchArgs A;
       MCFL_STCHK(E.L.prepare(c, jd, mt, perjob, uni, &A, E.stream));
+      s->t_build += since(t_build0);
+      const auto t_launch0 = std::chrono::steady_clock::now();
       plan_tiles(c, A, jd.data(), jd.size(), 0, perjob ? 1 : 0);  // an LDS fhd tile per request tallying fhd
       MCFL_STCHK(E.L.upload(jd, E.stream));
-      const int nblocks = grid_blocks(c, A, 0, perjob ? 1 : 0);
+      // leave `reserve` workgroup slots free, so the copy-outs and sums queued
+      // behind a finished epoch run beside the one still in flight
+      const int nblocks = std::max(1, grid_blocks(c, A, 0, perjob ? 1 : 0) - s->reserve);
       A.n_waves = (uint32_t)nblocks * (mcfl_wg_of(&A) / 64u);
       MCFL_HIPCHK(hipMemsetAsync(A.counter, 0, 8 * jd.size(), E.stream));
       MCFL_HIPCHK(hipEventRecord(E.e0, E.stream));
       if (mcfl_transport_launch(&A, 0, perjob ? 1 : 0, nblocks, E.stream) != 0) return MCFL_ERR_DEVICE;
       MCFL_HIPCHK(hipEventRecord(E.e1, E.stream));
+      s->t_launch += since(t_launch0);
       E.photons = A.total;
       E.busy = true;
       s->inflight.push_back(s->next);
@@ -854,19 +1047,37 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
     for (const auto& pt : O.parts)
       for (std::unique_ptr<Request>& qp : s->active)
         if (qp->id == pt.first) qp->finished += pt.second;
-    std::vector<std::unique_ptr<Request>> still;
-    for (std::unique_ptr<Request>& q : s->active) {
-      if ((q->phase == 1 || q->phase == 2) && q->finished >= q->r.photons) MCFL_STCHK(advance(s, *q, O.stream));
-      if (q->phase == 3) {
-        if (done_ids && nd < max_done) done_ids[nd] = q->id;
-        ++nd;
-        s->done[q->id] = std::move(q);
-      } else {
-        still.push_back(std::move(q));
+    for (std::unique_ptr<Request>& q : s->active)
+      if ((q->phase == 1 || q->phase == 2) && !q->pending && q->finished >= q->r.photons) {
+        const auto ta = std::chrono::steady_clock::now();
+        MCFL_STCHK(start_copy_out(s->hpool, *q, O.stream));
+        s->t_retire += since(ta);
       }
+    // the copies just queued normally land within a millisecond (they run
+    // beside the epoch in flight): give them that long, so this step can
+    // already report the requests they finish
+    const auto tq = std::chrono::steady_clock::now();
+    for (;;) {
+      bool waiting = false;
+      for (const std::unique_ptr<Request>& q : s->active)
+        if (q->pending && hipEventQuery(q->ev) == hipErrorNotReady) waiting = true;
+      if (!waiting || since(tq) > 2.0) break;
     }
-    s->active.swap(still);
+    MCFL_STCHK(finish_landed(false));
   }
+  // Nothing on the device but copy-outs: wait for them rather than spin.
+  if (s->inflight.empty()) MCFL_STCHK(finish_landed(true));
+  std::vector<std::unique_ptr<Request>> still;
+  for (std::unique_ptr<Request>& q : s->active) {
+    if (q->phase == 3) {
+      if (done_ids && nd < max_done) done_ids[nd] = q->id;
+      ++nd;
+      s->done[q->id] = std::move(q);
+    } else {
+      still.push_back(std::move(q));
+    }
+  }
+  s->active.swap(still);
   if (n_done) *n_done = nd;
   s->host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count() - t_wait;
   return MCFL_OK;

@@ -316,10 +316,12 @@ typedef struct mcfl_request_result {
   uint64_t epochs; /* epochs the request took part in */
 } mcfl_request_result;
 
-/* Each epoch advances every active request by an equal quantum in one launch:
-   an epoch target of 1e7 photons split over them, at least epoch_photons.
-   Admission is FIFO within mem_budget_bytes.  Two epochs are in flight at a
-   time on two streams. */
+/* Each epoch is one launch of about 2e7 photons (env MCFL_EPOCH_TARGET): every
+   active request gets at least epoch_photons of its phase, the rest of the
+   target goes to the requests with the least work left first.  Admission is
+   FIFO within mem_budget_bytes.  Two epochs are in flight at a time on two
+   streams; an epoch leaves 8 workgroup slots free (env MCFL_SERVER_RESERVE)
+   so that finished requests' tallies are copied out beside it. */
 mcfl_status mcfl_server_create(mcfl_ctx* ctx, uint64_t mem_budget_bytes, uint64_t epoch_photons, mcfl_server** out);
 /* Makes a medium resident; the simulation's bulks/labels/detector are copied. */
 mcfl_status mcfl_server_add_medium(mcfl_server* s, const mcfl_sim* medium, int32_t* medium_id);

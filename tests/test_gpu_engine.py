@@ -96,8 +96,9 @@ def test_importance_sampled_emission_matches_voxel_loop(ctx):
     assert abs(tot_is - tot_ref) < 5 * sig + 0.02 * tot_ref, (tot_is, tot_ref, sig)
 
 
-def test_server_results_equal_alone(ctx):
+def test_server_results_equal_alone(ctx, monkeypatch):
     sims = [medium(20, 20, 10, 0), medium(12, 16, 6, 1, n_below=1.0)]
+    monkeypatch.setenv("MCFL_EPOCH_TARGET", "4000")  # small epochs: every request is sliced over several
     srv = M.Server(ctx, mem_budget=1 << 30, epoch_photons=3000)
     mids = [srv.add_medium(s) for s in sims]
     reqs = [(mids[0], 5000, 11, (0.0, 0.0, 0.0), False), (mids[1], 2000, 12, (0.1, 0.0, 0.0), True),
@@ -126,6 +127,8 @@ def test_server_results_equal_alone(ctx):
         r = ctx.run_excitation(sim, seed=seed)
         assert np.array_equal(out["Rd"], r.Rd) and np.array_equal(out["Tt"], r.Tt), k
         assert out["totals"] == r.totals
+        if n > 3000:
+            assert out["epochs"] > (2 if fl else 1)
         if fl:
             assert out["epochs"] > 1
             Fx = r.fhd.astype(np.float64) / (4294967295.0 * r.totals["photons"])
