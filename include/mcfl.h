@@ -180,7 +180,8 @@ typedef struct mcfl_tallies {
 
 /* The device part of DoOneSimulation (CUDAMCFLmain.cu:60-167): runs one
  * excitation simulation to completion in a single persistent launch and
- * returns the integer tallies.  Outputs are overwritten. */
+ * returns the integer tallies.  Outputs are overwritten.  number_of_photons
+ * = 0 is MCFL_ERR_ARG (the reference's parser never produces it). */
 mcfl_status mcfl_run_excitation(mcfl_ctx* ctx, const mcfl_sim* sim, const mcfl_rng* rng, mcfl_tallies* out);
 
 /* DoOneSimulation, CUDAMCFLmain.cu:60-193: reference lane streams (x, a of

@@ -158,3 +158,59 @@ def test_layered_batch_is_refused(ctx):
     sim = layered3(1000)
     with pytest.raises(M.McflError):
         M.Batch(ctx, [sim], [M.BatchJob(0, 1000, 1)])
+
+
+def test_zero_and_one_photon(ctx):
+    """An empty run is refused as an argument error (the reference's parser
+    never yields one); a single photon gives the oracle's tallies."""
+    sim = small_sim(photons=0)
+    print("case: 0 photons", flush=True)
+    with pytest.raises(M.McflError):
+        ctx.run_excitation(sim, seed=5, sq=True)
+    print("case: 1 photon", flush=True)
+    sim.s.number_of_photons = 1
+    g = ctx.run_excitation(sim, seed=5, sq=True)
+    o = O.FromProduct(sim, sq=True)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(5, 0), 0, 1)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert g.totals["photons"] == 1 and balance(g.totals) == 0
+
+
+def test_single_voxel_medium(ctx):
+    """A 1x1x1 grid: every lateral step leaves the grid (medium 1 outside,
+    CUDAMCFLtransport.cu:319-333) and the fhd has one entry."""
+    lab = np.ones((1, 1, 1), np.uint8)
+    sim = M.Simulation.voxel(lab, MATS3[:1], esp=0.1, grid_size=10, det=(0.05, 0.05, 8, 8, 0.0, 0.0), photons=5000,
+                             n_above=1.0, n_below=1.33, fhd=True)
+    print("case: 1x1x1 medium, 5000 photons", flush=True)
+    g = ctx.run_excitation(sim, seed=8, axyz=True)
+    o = O.FromProduct(sim, axyz=True)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(8, 0), 0, 5000)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert g.fhd[0] > 0 and balance(g.totals) == 0
+
+
+def test_largest_material_table(ctx):
+    """98 materials (the reference's limit, labels 1..98 plus the two outer
+    media) with varied n, g and a glass material: bit-exact vs the oracle."""
+    rng = np.random.default_rng(17)
+    nb = 98
+    mats = []
+    for i in range(nb):
+        row = [float(rng.choice([1.33, 1.4, 1.45])), float(rng.uniform(0.05, 2.0)), float(rng.uniform(20, 120)),
+               float(rng.uniform(0, 1)), 0.2, 0.1, 0.1, float(rng.uniform(0.0, 0.95))]
+        if i == 40:  # glass (mus = 0, non-absorbing: the MoveToFirstBoundary path)
+            row[1], row[2], row[3] = 0.0, 0.0, 0.0
+        mats.append(row)
+    lab = rng.integers(1, nb + 1, size=(8, 16, 16)).astype(np.uint8)
+    sim = M.Simulation.voxel(lab, mats, esp=0.8, grid_size=10, det=(0.1, 0.1, 16, 16, 0.0, 0.0), photons=6000,
+                             n_above=1.0, n_below=1.2, fhd=True)
+    print("case: 98 materials, 16x16x8, 6000 photons", flush=True)
+    g = ctx.run_excitation(sim, seed=9)
+    o = O.FromProduct(sim)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(9, 0), 0, 6000)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert balance(g.totals) == 0
