@@ -69,6 +69,32 @@ def inclusion_simulation(photons=100_000_000, seed=2024, fhd=True, source=(0.0, 
                             fhd=fhd)
 
 
+def sphere_simulation(photons=1_000_000, fhd=True, photons_per_voxel=40000):
+    """The small plain problem in the reference's own grid shape: esp = 2 cm,
+    grid_size 10 /cm, so the grid is (4 esp gs)^2 x (esp gs) = 80x80x20 voxels
+    of 1 mm (CUDAMCFLio.cu:609-612); background mua 0.1, mus 100 /cm, g 0.9,
+    n 1.4; a voxelized sphere of radius 0.5 cm centred at depth 1.0 cm with mua
+    0.5 /cm; fluorophore absorption 0.01 /cm (background) and 0.1 /cm (sphere),
+    quantum yield 0.1, emission absorption 0.08 /cm (AbsCEmi = (0.08 - mua) /
+    flc); ambient n 1.0; 40x40 images of 1 mm pixels; pencil beam at the
+    centre."""
+    esp, gs = 2.0, 10
+    nx = ny = int(4 * esp * gs)
+    nz = int(esp * gs)
+    z, y, x = np.ogrid[0:nz, 0:ny, 0:nx]
+    # voxel centres in cm: x from -2 esp, z from 0
+    xc = (x + 0.5) / gs - 2 * esp
+    yc = (y + 0.5) / gs - 2 * esp
+    zc = (z + 0.5) / gs
+    lab = np.ones((nz, ny, nx), np.uint8)
+    lab[xc * xc + yc * yc + (zc - 1.0) ** 2 < 0.5 * 0.5] = 2
+    mats = [[1.4, 0.1, 100.0, 1.0, 0.01, 0.08 - 0.1, 0.1, 0.9],
+            [1.4, 0.5, 100.0, 1.0, 0.1, 0.08 - 0.5, 0.1, 0.9]]
+    return Simulation.voxel(lab, mats, esp=esp, grid_size=gs, det=(0.1, 0.1, 40, 40, 0.0, 0.0), photons=photons,
+                            src_kind=SOURCE_PENCIL, n_above=1.0, n_below=1.0, fhd=fhd, reference_grid=True,
+                            photons_per_voxel=photons_per_voxel)
+
+
 def write_bulk(path, labels):
     """Bulk-descriptor file: labels x fastest, '%i' separated by spaces, one line."""
     flat = np.asarray(labels, np.uint8).reshape(-1)  # (nz, ny, nx) C-order == x fastest

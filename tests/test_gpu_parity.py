@@ -214,3 +214,27 @@ def test_largest_material_table(ctx):
     assert_same(g, r)
     check_totals(g.totals, r.tot)
     assert balance(g.totals) == 0
+
+
+def test_reference_shaped_sphere_problem_bit_exact(ctx):
+    """The small plain problem in the reference's own grid shape (80x80x20 of
+    1 mm, a voxelized sphere): excitation and the every-voxel fluorescence
+    stage bit-exact vs the oracle."""
+    from cudamcfl_amd.media import sphere_simulation
+    sim = sphere_simulation(photons=30000)
+    print("case: 80x80x20 sphere problem, 30000 photons, seed 12", flush=True)
+    g = ctx.run_excitation(sim, seed=12)
+    o = O.FromProduct(sim)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(12, 0), 0, 30000)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    print("case: its fluorescence stage, 1 photon per emitting voxel", flush=True)
+    Fx = g.fhd.astype(np.float64) / (0xFFFFFFFF * g.totals["photons"])
+    fl = ctx.run_fluorescence(sim, Fx, 1, seed=13)
+    vox, scale, unit = M.fl_plan(sim, Fx, 1)
+    assert fl["voxels"] == len(vox) > 0
+    keys = np.array([M.stream_key(13, int(v)) for v in vox], np.uint64)
+    oe = O.FromProduct(sim, emission=True)
+    re = O.run_fluor_jobs(oe.o, a_tab(), vox, keys, scale, 1)
+    assert np.array_equal(fl["raw_R"], re.Rd) and np.array_equal(fl["raw_T"], re.Tt)
+    check_totals(fl["totals"], re.tot)

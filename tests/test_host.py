@@ -249,3 +249,19 @@ def test_output_file_readers_round_trip(tmp_path):
     pos, hist = read_time(tmp_path / "rt_time.dat")
     assert pos.shape == (2, 2) and hist.shape == (2, 3)
     assert np.allclose(hist[1], r.time[[1, 3, 5]].astype(np.float64) / scale, rtol=1e-6)
+
+
+def test_reference_shaped_sphere_problem():
+    """80x80x20 voxels of 1 mm from esp = 2 cm and grid_size 10 (CUDAMCFLio.cu:609-612),
+    a voxelized sphere of radius 5 voxels, the parser's derived optics."""
+    import numpy as np
+    from cudamcfl_amd.media import sphere_simulation
+    sim = sphere_simulation(photons=1000)
+    s = sim.s
+    assert (s.nx, s.ny, s.nz) == (80, 80, 20) and abs(s.halfx - 4.0) < 1e-6
+    counts = np.bincount(sim.labels.ravel(), minlength=3)
+    assert abs(counts[2] - 4.0 / 3.0 * np.pi * 5 ** 3) < 0.1 * 4.0 / 3.0 * np.pi * 5 ** 3
+    b = s.bulks
+    assert abs(b[1].mua - 0.11) < 1e-6 and abs(b[2].mua - 0.6) < 1e-6        # mua + flc AbsCExi
+    assert abs(b[1].muaf - 0.08) < 1e-6 and abs(b[2].muaf - 0.08) < 1e-6      # mua + flc AbsCEmi
+    assert abs(b[1].mutr - 1.0 / 100.1) < 1e-7 and abs(b[2].mutr - 1.0 / 100.5) < 1e-7
