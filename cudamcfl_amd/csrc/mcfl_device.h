@@ -50,7 +50,10 @@ enum {
   MCFL_DT_ABORTED
 };
 
-/* Per-material constants (32 bytes, LDS resident).  twog == 0 <=> g == 0. */
+/* Per-material constants (48 bytes, LDS resident).  twog == 0 <=> g == 0.
+   The surface quotients are the two divisions of Reflect against the media
+   above (material 0) and below (the table's last outer material), rounded as
+   the kernel would round them (n*n)/(n'*n') and n/n'. */
 struct MatDev {
   float mutr;     /* 1/(mua_base + mus), FLT_MAX for glass */
   float mua_mutr; /* mua*mutr: absorbed fraction per interaction */
@@ -60,6 +63,8 @@ struct MatDev {
   float onepg2;   /* 1 + g^2 */
   float twog;     /* 2g */
   float inv2g;    /* 1/(2g), 0 when g == 0 */
+  float q_top, r_top; /* (n*n)/(n0*n0), n/n0 */
+  float q_bot, r_bot; /* the same against the medium below */
 };
 
 struct MediumDev {
@@ -160,7 +165,7 @@ struct LaunchArgs {
 static inline uint32_t mcfl_wg_of(const LaunchArgs* A) { return A->wg ? A->wg : (uint32_t)MCFL_WG; }
 
 static inline uint32_t mcfl_lds_bytes(uint32_t nslots, uint32_t tile_n, uint32_t n_mats, uint32_t n_zb) {
-  return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * 32u + n_zb * 8u;
+  return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * (uint32_t)sizeof(MatDev) + n_zb * 8u;
 }
 
 #endif

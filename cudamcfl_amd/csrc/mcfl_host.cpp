@@ -58,6 +58,17 @@ static void mat_of(float mutr, float mua, float g, float n, MatDev& m) {
   m.inv2g = (g != 0.0f) ? 1.0f / (2.0f * g) : 0.0f;
 }
 
+// Reflect's quotients against the outer media (material 0 above, `last` below).
+static void surface_quotients(std::vector<MatDev>& out, uint32_t last) {
+  const float n0 = out[0].n, nl = out[last].n;
+  for (MatDev& m : out) {
+    m.q_top = (m.n * m.n) / (n0 * n0);
+    m.r_top = m.n / n0;
+    m.q_bot = (m.n * m.n) / (nl * nl);
+    m.r_bot = m.n / nl;
+  }
+}
+
 // Layered medium (bulk method 1): layers 0..n_layers+1, then the inclusion.
 void build_mats(const mcfl_sim* s, bool emission, std::vector<MatDev>& out) {
   if (s->bulk_method == 1) {
@@ -69,6 +80,7 @@ void build_mats(const mcfl_sim* s, bool emission, std::vector<MatDev>& out) {
     }
     const mcfl_inclusion& c = s->inclusion;
     mat_of(c.mutr, emission ? c.muaf : c.mua, c.g, c.n, out[n]);
+    surface_quotients(out, n - 1);
     return;
   }
   const uint32_t n = s->n_bulks + 2;
@@ -77,6 +89,7 @@ void build_mats(const mcfl_sim* s, bool emission, std::vector<MatDev>& out) {
     const mcfl_bulk& b = s->bulks[i];
     mat_of(b.mutr, emission ? b.muaf : b.mua, b.g, b.n, out[i]);
   }
+  surface_quotients(out, n - 1);
 }
 
 // Layer bounds (z_min, z_max) of a layered medium, 0 above and n_layers+1 below.

@@ -172,9 +172,14 @@ __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint3
   p.dz = nz * r;
 }
 
-/* Reflect, CUDAMCFLtransport.cu:628-700 (bulk method 2). */
-__device__ __forceinline__ void reflect(const MatDev* mats, Ph& p, uint32_t nb, uint64_t& x, uint32_t a) {
-  const float n1 = mats[p.lab].n, n2 = mats[nb].n;
+/* Reflect, CUDAMCFLtransport.cu:628-700 (bulk method 2).  With PRE, against
+   the outer media (nb 0 or `last`) the two quotients come precomputed with
+   the same rounding (the variant with the most live registers divides). */
+template <bool PRE>
+__device__ __forceinline__ void reflect(const MatDev* mats, Ph& p, uint32_t nb, uint32_t last, uint64_t& x,
+                                        uint32_t a) {
+  const MatDev& m1 = mats[p.lab];
+  const float n1 = m1.n, n2 = mats[nb].n;
   const float ci = mcfl_fabs(p.dz);
   if (n1 == n2) {
     p.lab = nb;
@@ -193,7 +198,8 @@ __device__ __forceinline__ void reflect(const MatDev* mats, Ph& p, uint32_t nb, 
       p.lab = nb;
     return;
   }
-  float e = (n1 * n1) / (n2 * n2) * (1.0f - ci2);
+  const float q = PRE && nb == 0u ? m1.q_top : (PRE && nb == last ? m1.q_bot : (n1 * n1) / (n2 * n2));
+  float e = q * (1.0f - ci2);
   float r = 2.0f * mcfl_sqrt(mcfl_fmax(0.0f, (1.0f - ci2) * (1.0f - e) * e * ci * ci));
   e = e + ci2 * (1.0f - 2.0f * e);
   r = e * ((1.0f - e - r) / ((1.0f - e + r) * (e + r)));
@@ -201,7 +207,7 @@ __device__ __forceinline__ void reflect(const MatDev* mats, Ph& p, uint32_t nb, 
     p.dz = -p.dz;
     return;
   }
-  r = n1 / n2;
+  r = PRE && nb == 0u ? m1.r_top : (PRE && nb == last ? m1.r_bot : n1 / n2);
   e = r * r * (1.0f - ci2);
   p.dx *= r;
   p.dy *= r;
@@ -561,7 +567,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       nb = (uint32_t)nl;
       if (nb != L) { /* :141-145: no absorption or scattering after a layer change */
         s = 0.0f;
-        reflect(mats, p, nb, x, a);
+        reflect<!(TEMPORAL && PERJOB)>(mats, p, nb, m.last, x, a);
       }
       mm = in_sphere(m, p.x, p.y, p.z) ? mats[m.n_layers + 2u] : mats[p.lab]; /* :200-224 */
     } else {
@@ -590,7 +596,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       if (nb == p.lab && mc.mutr != kFltMax) nb = in ? (uint32_t)m.labels[vi] : 1u; /* :319-333 */
       mm = mc; /* material after a possible interface crossing */
       if (nb != p.lab) {
-        reflect(mats, p, nb, x, a);
+        reflect<!(TEMPORAL && PERJOB)>(mats, p, nb, m.last, x, a);
         mm = mats[p.lab];
       }
     }
@@ -615,8 +621,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     }
 
     if (p.lab == 0u || p.lab == m.last) {
-      const JobDev& J = A.jobs[job];
-      exit_tally<TEMPORAL>(J, p, p.lab == 0u, (flags & MCFL_F_SCALED) != 0u, escale, st);
+      exit_tally<TEMPORAL>(A.jobs[job], p, p.lab == 0u, (flags & MCFL_F_SCALED) != 0u, escale, st);
       p.w = 0u;
       s = 0.0f;
     }
