@@ -63,6 +63,7 @@ MCFL_HD float mcfl_bitsf(uint32_t u) {
 }
 
 MCFL_HD float mcfl_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+MCFL_HD double mcfl_fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 MCFL_HD float mcfl_sqrt(float x) { return __builtin_sqrtf(x); }
 MCFL_HD float mcfl_rint(float x) { return __builtin_rintf(x); }
 MCFL_HD float mcfl_ceil(float x) { return __builtin_ceilf(x); }
@@ -161,6 +162,20 @@ MCFL_HD uint32_t mcfl_f2u_rn(float f) { return mcfl_f2u_rz(mcfl_rint(f)); }
 /* double -> uint64 toward zero with the same saturation rules. */
 MCFL_HD uint64_t mcfl_d2u64_rz(double d) {
   return (d >= 18446744073709551616.0) ? 0xFFFFFFFFFFFFFFFFull : ((d > 0.0) ? (uint64_t)d : 0ull);
+}
+
+/* Light speed in tissue units, cm per fs (CUDAMCFL.h:29). */
+#define MCFL_CCMFS 2.99792E-5
+
+/* q / MCFL_CCMFS for a float q (the time-of-flight increments, CUDAMCFLtransport.cu:316):
+   the product with the rounded reciprocal and one FMA correction of its residual
+   (Markstein's correction), three double operations instead of a division.  It
+   equals the correctly rounded quotient for every float q; the tests check all
+   2^31 non-negative finite floats (the formula is odd in q). */
+MCFL_HD double mcfl_tof_div(double q) {
+  const double r = 1.0 / MCFL_CCMFS;
+  const double y = q * r;
+  return mcfl_fmad(mcfl_fmad(-y, MCFL_CCMFS, q), r, y);
 }
 
 #endif /* MCFL_FMATH_H */

@@ -811,3 +811,17 @@ void orc_mwc_draws(uint64_t x, uint32_t a, size_t n, uint64_t* states, float* co
 }
 
 uint64_t orc_photon_state(uint64_t key, uint64_t k, uint32_t a) { return mcfl_photon_state(key, k, a); }
+
+/* Float bit patterns b in [lo, hi) for which the kernel's time-of-flight
+   quotient (mcfl_fmath.h mcfl_tof_div) differs from the IEEE division used
+   here (CUDAMCFLtransport.cu:316). */
+uint64_t orc_tof_div_mismatches(uint32_t lo, uint32_t hi) {
+  uint64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (int64_t b = (int64_t)lo; b < (int64_t)hi; ++b) {
+    const float f = mcfl_bitsf((uint32_t)b);
+    const double q = (double)f;
+    if (!(mcfl_tof_div(q) == q / C_CMFS)) ++bad;
+  }
+  return bad;
+}

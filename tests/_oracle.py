@@ -244,6 +244,8 @@ def _bind_extras(L):
     L.orc_mwc_draws.argtypes = [C.c_uint64, C.c_uint32, C.c_size_t, U64P, F32P]
     L.orc_photon_state.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
     L.orc_photon_state.restype = C.c_uint64
+    L.orc_tof_div_mismatches.argtypes = [C.c_uint32, C.c_uint32]
+    L.orc_tof_div_mismatches.restype = C.c_uint64
     L._extras = True
 
 
@@ -269,3 +271,9 @@ def photon_state(key, k, a):
     L = lib()
     _bind_extras(L)
     return int(L.orc_photon_state(key, k, a))
+
+
+def tof_div_mismatches(lo, hi):
+    L = lib()
+    _bind_extras(L)
+    return int(L.orc_tof_div_mismatches(lo, hi))

@@ -83,6 +83,16 @@ def test_sqrt_and_renormalisation_error_bounds():
     assert np.max(np.abs(got - ref)[near] / ulp[near]) <= 1.0
 
 
+def test_tof_quotient_is_the_ieee_division():
+    """The kernel's time-of-flight increment q / 2.99792e-5 (a product and one
+    FMA correction) equals the IEEE double quotient of the reference's
+    expression (CUDAMCFLtransport.cu:316) for every non-negative finite float q,
+    and for a stretch of negative ones (the formula is odd)."""
+    assert O.tof_div_mismatches(0, 0x7F800000) == 0
+    assert O.tof_div_mismatches(0x80000000, 0x80000000 + (1 << 26)) == 0
+    assert O.tof_div_mismatches(0xC0000000, 0xC0000000 + (1 << 26)) == 0
+
+
 def test_u32_to_float_toward_zero():
     rng = np.random.default_rng(2)
     u = np.concatenate([rng.integers(0, 2 ** 32, 100000, dtype=np.uint64),
