@@ -1,4 +1,4 @@
-"""The .mci / bulk-file parser (host C++) under AddressSanitizer and
+"""The .mci / bulk-file parser and the output writers (host C++) under AddressSanitizer and
 UndefinedBehaviorSanitizer: the reference's sample files and malformed
 variants of them (truncated at every line, every line removed or replaced by
 a negative, huge, non-numeric or empty value, short / out-of-range / missing
@@ -60,9 +60,10 @@ def test_parser_under_sanitizers(driver, tmp_path):
         files.append(p.name)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
     failures = []
-    for i in range(0, len(files), 64):  # several files per process
-        r = subprocess.run([driver] + files[i:i + 64], cwd=tmp_path, capture_output=True, text=True, env=env,
-                           timeout=300)
+    batches = [files[i:i + 64] for i in range(0, len(files), 64)]  # several files per process
+    batches.append(["-w"] + files[::25])  # the writers on a sample of the parsed files
+    for b in batches:
+        r = subprocess.run([driver] + b, cwd=tmp_path, capture_output=True, text=True, env=env, timeout=300)
         if (r.returncode != 0 or "runtime error" in r.stderr or "AddressSanitizer" in r.stderr
                 or "LeakSanitizer" in r.stderr):
             failures.append(r.stderr[-2000:])
