@@ -6,6 +6,10 @@
 //   -L        excitation on the reference's 32768 per-lane MWC streams
 //             (default: one stream per photon, full-GPU occupancy)
 //   -F<seed>  seed of the emission stage (the reference uses time(NULL))
+//   -I<N>     emission stage by importance sampling: N photons launched from
+//             voxels drawn in proportion to eY (1 - albedof) Fx, an unbiased
+//             estimate of the same FlR/FlT images as the every-voxel loop
+//             (photons per voxel x every emitting voxel) with a budget of N
 //   several .mci files: their excitation simulations run as batched launches
 //             (files sharing a medium share it on the device), then every
 //             file's outputs are written as if it had been run alone with the
@@ -24,9 +28,36 @@ namespace {
 
 struct Options {
   unsigned long long seed = 0, fl_seed = 0;
+  unsigned long long is_photons = 0;  // -I<N>: importance-sampled emission budget (0: every-voxel loop)
   bool lanes = false;
   int ignoreA = 0;
 };
+
+// Importance-sampled emission stage: one emission job of o.is_photons photons
+// on the medium of s; images scaled to the reference's FlR/FlT units.
+mcfl_status emission_is(mcfl_ctx* ctx, const mcfl_sim* s, const std::vector<double>& Fx, const Options& o,
+                        std::vector<double>& FlR, std::vector<double>& FlT, double* kernel_ms) {
+  const size_t np = (size_t)s->det.nx * (s->det.ny + 1), xy = (size_t)s->det.nx * s->det.ny;
+  std::vector<uint64_t> R(np), T(np);
+  mcfl_job j;
+  memset(&j, 0, sizeof j);
+  j.medium = 0;
+  j.src_kind = MCFL_SOURCE_EMISSION;
+  j.photons = o.is_photons;
+  j.seed = o.fl_seed;
+  j.Fx = Fx.data();
+  j.Rd_xy = R.data();
+  j.Tt_xy = T.data();
+  const mcfl_sim* media[1] = {s};
+  mcfl_status st = mcfl_run_batch(ctx, media, 1, &j, 1, kernel_ms);
+  if (st != MCFL_OK) return st;
+  const double k = j.fl_unit / ((double)s->det.dx * (double)s->det.dy);
+  for (size_t i = 0; i < xy; ++i) {
+    FlR[i] = (double)R[i] * k;
+    FlT[i] = (double)T[i] * k;
+  }
+  return MCFL_OK;
+}
 
 size_t nvox(const mcfl_sim* s) { return (size_t)s->nx * s->ny * s->nz; }
 size_t npix(const mcfl_sim* s) { return (size_t)s->det.nx * (s->det.ny + 1); }
@@ -54,6 +85,20 @@ bool finish_file(mcfl_ctx* ctx, mcfl_sim* s, std::vector<double>& Fx, const Opti
   printf("Flourescence simulation... \n");
   const size_t xy = (size_t)s->det.nx * s->det.ny;
   std::vector<double> FlR(xy), FlT(xy);
+  if (o.is_photons > 0 && s->bulk_method == 2) {  // (a method-1 medium emits nothing, CUDAMCFLmain.cu:520,535)
+    double ms = 0.0;
+    const mcfl_status st = emission_is(ctx, s, Fx, o, FlR, FlT, &ms);
+    if (st != MCFL_OK) {
+      printf("Fluorescence failed: %s\n", mcfl_status_string(st));
+      return false;
+    }
+    *fl_photons = o.is_photons;
+    printf("\n\nFlourescence simulation finished! (importance-sampled emission, %llu photons, kernel %.1f ms)\n",
+           (unsigned long long)o.is_photons, ms);
+    printf("Writing results files...\n");
+    mcfl_write_fl(s, FlR.data(), FlT.data());
+    return true;
+  }
   mcfl_fl_result res;
   memset(&res, 0, sizeof res);
   mcfl_status st =
@@ -255,6 +300,10 @@ int main(int argc, char* argv[]) {
     }
     if (!strncmp(argv[i], "-F", 2)) {
       o.fl_seed = strtoull(argv[i] + 2, nullptr, 10);
+      continue;
+    }
+    if (!strncmp(argv[i], "-I", 2)) {
+      o.is_photons = strtoull(argv[i] + 2, nullptr, 10);
       continue;
     }
     args.push_back(argv[i]);

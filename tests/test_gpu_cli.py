@@ -72,3 +72,34 @@ def test_cli_several_files_equal_single_runs(tmp_path):
             a = (tmp_path / "many" / (name + suffix)).read_bytes()
             b = (tmp_path / "one" / (name + suffix)).read_bytes()
             assert a == b, (name, suffix)
+
+
+def test_cli_importance_sampled_emission(tmp_path):
+    """`mcfl f.mci -I<N>` estimates the same fluorescence images as the
+    every-voxel loop: the FlR totals agree within Monte Carlo noise."""
+    import numpy as np
+    from media import MATS3, small_labels
+    from cudamcfl_amd.media import write_bulk, write_mci
+    from cudamcfl_amd.files import read_image
+    lab = small_labels(20, 20, 10, 2)
+    nz, ny, nx = lab.shape
+    for d in ("loop", "is"):
+        (tmp_path / d).mkdir()
+        write_bulk(tmp_path / d / "a.bulk", lab)
+        write_mci(tmp_path / d / "f.mci", "f", "a.bulk", MATS3, esp=nz / 10, grid_size=10, photons=200000,
+                  photons_per_voxel=33000, do_fl=1, det=(0.1, 0.1, nx, ny, 0.0, 0.0), n_below=1.33, dims=(nx, ny, nz))
+    print("case: mcfl CLI, every-voxel emission vs -I4000000", flush=True)
+    r = subprocess.run([str(EXE), "f.mci", "-S5", "-F9"], cwd=tmp_path / "loop", capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    r = subprocess.run([str(EXE), "f.mci", "-S5", "-F9", "-I4000000"], cwd=tmp_path / "is", capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "importance-sampled emission" in r.stdout
+    a = read_image(str(tmp_path / "loop" / "f_FlR.dat"))
+    b = read_image(str(tmp_path / "is" / "f_FlR.dat"))
+    assert a.shape == b.shape and a.sum() > 0 and b.sum() > 0
+    assert abs(a.sum() - b.sum()) < 0.03 * a.sum(), (a.sum(), b.sum())
+    # the excitation outputs do not depend on how the emission stage is run
+    for suffix in ("_Refl.dat", "_Trans.dat", "_PHD-Ascii.dat"):
+        assert (tmp_path / "loop" / ("f" + suffix)).read_bytes() == (tmp_path / "is" / ("f" + suffix)).read_bytes()
