@@ -91,7 +91,8 @@ _lib = None
 EXPORTS = ["mcfl_read_simulation_data", "mcfl_free_simulations", "mcfl_interpret_arg", "mcfl_init_rng",
            "mcfl_default_primes_path", "mcfl_stream_key", "mcfl_ctx_create", "mcfl_ctx_destroy",
            "mcfl_status_string", "mcfl_ctx_info", "mcfl_run_excitation", "mcfl_DoOneSimulation", "mcfl_fl_plan",
-           "mcfl_run_fluorescence", "mcfl_write_simulation_results", "mcfl_write_phd", "mcfl_write_fl"]
+           "mcfl_run_fluorescence", "mcfl_run_fluorescence_lanes", "mcfl_write_simulation_results", "mcfl_write_phd",
+           "mcfl_write_fl"]
 
 
 def lib():
@@ -122,6 +123,8 @@ def lib():
     L.mcfl_fl_plan.argtypes = [C.POINTER(Sim), F64P, C.c_uint64, U32P, F64P, U64P, F64P]
     L.mcfl_run_fluorescence.argtypes = [C.c_void_p, C.POINTER(Sim), F64P, C.c_uint64, C.c_uint64, F64P, F64P,
                                         C.POINTER(FlResult)]
+    L.mcfl_run_fluorescence_lanes.argtypes = [C.c_void_p, C.POINTER(Sim), F64P, C.c_uint64, C.POINTER(Rng), F64P, F64P,
+                                              C.POINTER(FlResult)]
     L.mcfl_write_simulation_results.argtypes = [C.POINTER(Sim), C.POINTER(Tallies), C.c_double]
     L.mcfl_write_phd.argtypes = [C.POINTER(Sim), F64P]
     L.mcfl_write_fl.argtypes = [C.POINTER(Sim), F64P, F64P]
@@ -404,6 +407,24 @@ class Context:
             rng.mode, rng.seed = RNG_PHOTON, int(seed if seed is not None else 1)
         check(lib().mcfl_run_excitation(self.h, C.byref(sim.s), C.byref(rng), C.byref(r.t)), "run_excitation")
         return r
+
+    def run_fluorescence_lanes(self, sim, Fx, n_per_voxel, lanes):
+        """mcfl_run_fluorescence_lanes: the voxel loop on the reference's lane streams
+        lanes=(x, a) (x advanced in place)."""
+        x, a = lanes
+        xy = sim.s.det.nx * sim.s.det.ny
+        FlR, FlT = np.zeros(xy), np.zeros(xy)
+        res = FlResult()
+        rawR, rawT = np.zeros(sim.npix, np.uint64), np.zeros(sim.npix, np.uint64)
+        res.FlR_raw, res.FlT_raw = _p(rawR, U64P), _p(rawT, U64P)
+        rng = Rng()
+        rng.mode, rng.x, rng.a, rng.n_lanes = RNG_LANES, _p(x, U64P), _p(a, U32P), len(a)
+        Fx = np.ascontiguousarray(Fx, np.float64)
+        check(lib().mcfl_run_fluorescence_lanes(self.h, C.byref(sim.s), _p(Fx, F64P), n_per_voxel, C.byref(rng),
+                                                _p(FlR, F64P), _p(FlT, F64P), C.byref(res)), "run_fluorescence_lanes")
+        return {"FlR": FlR, "FlT": FlT, "raw_R": rawR, "raw_T": rawT, "unit": res.unit, "voxels": res.voxels,
+                "photons": res.photons, "kernel_ms": res.kernel_ms,
+                "totals": {n: int(res.totals[i]) for i, n in enumerate(TOT_NAMES)}}
 
     def run_fluorescence(self, sim, Fx, n_per_voxel, seed):
         xy = sim.s.det.nx * sim.s.det.ny

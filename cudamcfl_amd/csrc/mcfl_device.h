@@ -35,7 +35,7 @@
 #define MCFL_WG_ALT2 1024
 #define MCFL_TALLY_COPIES 8      /* per-step volume tallies: one copy per XCD, summed after the launch */
 
-enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2 };
+enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2, MCFL_WORK_VLANES = 3 };
 enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2, MCFL_SRC_EMISSION = 3 };
 enum {
   MCFL_DT_LAUNCHED_W = 0,
@@ -145,10 +145,13 @@ struct LaunchArgs {
   const double* vscale;
   uint64_t per_job;
   uint64_t vseed;
-  /* MCFL_WORK_LANES: lane i runs q (+1 if i < rem) photons on (xs[i], as[i]) */
+  /* MCFL_WORK_LANES: lane i runs q (+1 if i < rem) photons on (xs[i], as[i]);
+     MCFL_WORK_VLANES: lane i runs that many photons from each voxel vox[0..n_vox)
+     in turn (the reference's emission loop), exits scaled by vscale[j] */
   unsigned long long* xs;
   const uint32_t* as;
   uint64_t lane_q, lane_rem;
+  uint32_t n_vox;
   /* LDS tile of a workgroup's home job's photon hitting density: voxels
      [t0, t0+tn) per axis, t0 per job (tz0 = 0) */
   uint32_t tile_n; /* entries (0: no tile) */

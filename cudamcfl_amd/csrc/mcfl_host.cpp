@@ -444,11 +444,12 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   if (const char* e = getenv("MCFL_REFILL_MIN")) c->refill_min = (uint32_t)std::min(64, std::max(1, atoi(e)));
   if (const char* e = getenv("MCFL_CHUNK_MAX")) c->chunk_max = (uint32_t)std::min(1 << 20, std::max(64, atoi(e)));
   snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
-  for (int w = 0; w < 3; ++w)
+  for (int w = 0; w < 4; ++w)
     for (int t = 0; t < 2; ++t)
       for (int pj = 0; pj < 2; ++pj)
         for (int ly = 0; ly < 2; ++ly) {
           if (ly && (pj || w == MCFL_WORK_VOXELS)) continue;  // no such variant
+          if (w == MCFL_WORK_VLANES && (t || pj || ly)) continue;
           if (mcfl_transport_occupancy(w, t, pj, ly, 0, 0, &c->bpc[w][t][pj][ly]) != 0) return fail(MCFL_ERR_DEVICE);
         }
   if (prop.maxSharedMemoryPerMultiProcessor > 0) c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
@@ -691,10 +692,18 @@ extern "C" mcfl_status mcfl_fl_plan(const mcfl_sim* s, const double* Fx, uint64_
   return n <= cap ? MCFL_OK : MCFL_ERR_CAPACITY;
 }
 
-extern "C" mcfl_status mcfl_run_fluorescence(mcfl_ctx* c, const mcfl_sim* s, const double* Fx, uint64_t nper,
-                                             uint64_t seed, double* FlR, double* FlT, mcfl_fl_result* res) {
+// The emission stage on photon streams (rng == nullptr: one job per emitting
+// voxel, MCFL_WORK_VOXELS) or on the reference's lanes (MCFL_WORK_VLANES:
+// voxels in the reference loop's order, lane states advanced in place).
+static mcfl_status fluorescence(mcfl_ctx* c, const mcfl_sim* s, const double* Fx, uint64_t nper, uint64_t seed,
+                                const mcfl_rng* rng, double* FlR, double* FlT, mcfl_fl_result* res) {
   if (!c || !s || !Fx || nper == 0) return MCFL_ERR_ARG;
   STCHK(validate(s));
+  const bool lanes = rng != nullptr;
+  if (lanes) {
+    if (!rng->x || !rng->a || rng->n_lanes == 0) return MCFL_ERR_ARG;
+    if (nper < rng->n_lanes) return MCFL_ERR_PHOTONS;  // DoOneSimulationFl fails (CUDAMCFLmain.cu:268-272)
+  }
   if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
   const size_t nv = nvox_of(s), npix = npix_of(s);
   if (s->bulk_method == 1) {  // no emitting voxel (see mcfl_fl_plan): zero images
@@ -716,6 +725,28 @@ extern "C" mcfl_status mcfl_run_fluorescence(mcfl_ctx* c, const mcfl_sim* s, con
   uint64_t nvx = nv;
   double unit = 1.0;
   STCHK(mcfl_fl_plan(s, Fx, nper, vox.data(), scale.data(), &nvx, &unit));
+  vox.resize(nvx);
+  scale.resize(nvx);
+  if (lanes) {  // the reference loop's order: x outermost, then y, then z (CUDAMCFLmain.cu:471-473)
+    std::vector<size_t> ord(nvx);
+    for (size_t j = 0; j < nvx; ++j) ord[j] = j;
+    const uint32_t nx = (uint32_t)s->nx, nxy = (uint32_t)s->nx * (uint32_t)s->ny;
+    std::sort(ord.begin(), ord.end(), [&](size_t p, size_t q) {
+      const uint32_t u = vox[p], w = vox[q];
+      const uint32_t ux = u % nx, wx = w % nx, uy = (u % nxy) / nx, wy = (w % nxy) / nx;
+      if (ux != wx) return ux < wx;
+      if (uy != wy) return uy < wy;
+      return u / nxy < w / nxy;
+    });
+    std::vector<uint32_t> v2(nvx);
+    std::vector<double> s2(nvx);
+    for (size_t j = 0; j < nvx; ++j) {
+      v2[j] = vox[ord[j]];
+      s2[j] = scale[ord[j]];
+    }
+    vox.swap(v2);
+    scale.swap(s2);
+  }
 
   LaunchArgs A;
   memset(&A, 0, sizeof A);
@@ -753,8 +784,20 @@ extern "C" mcfl_status mcfl_run_fluorescence(mcfl_ctx* c, const mcfl_sim* s, con
 
   A.mats = static_cast<const MatDev*>(c->mats.p);
   A.n_mats = (uint32_t)mats.size();
-  A.work = MCFL_WORK_VOXELS;
-  A.total = nvx * nper;
+  A.work = lanes ? MCFL_WORK_VLANES : MCFL_WORK_VOXELS;
+  A.total = lanes ? (nvx ? rng->n_lanes : 0) : nvx * nper;
+  if (lanes) {
+    const size_t L = rng->n_lanes;
+    STCHK(ensure(c->xs, L * 8));
+    STCHK(ensure(c->as, L * 4));
+    HIPCHK(hipMemcpyAsync(c->xs.p, rng->x, L * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->as.p, rng->a, L * 4, hipMemcpyHostToDevice, c->stream));
+    A.xs = static_cast<unsigned long long*>(c->xs.p);
+    A.as = static_cast<const uint32_t*>(c->as.p);
+    A.lane_q = nper / L;
+    A.lane_rem = nper % L;
+    A.n_vox = (uint32_t)nvx;
+  }
   A.counter = static_cast<unsigned long long*>(c->counter.p);
   A.a_tab = c->d_atab;
   A.a_mask = kTablePhotons - 1;
@@ -785,5 +828,17 @@ extern "C" mcfl_status mcfl_run_fluorescence(mcfl_ctx* c, const mcfl_sim* s, con
     res->kernel_ms = ms;
     HIPCHK(hipMemcpy(res->totals, c->tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost));
   }
+  if (lanes && A.total > 0) HIPCHK(hipMemcpy(rng->x, c->xs.p, (size_t)rng->n_lanes * 8, hipMemcpyDeviceToHost));
   return MCFL_OK;
+}
+
+extern "C" mcfl_status mcfl_run_fluorescence(mcfl_ctx* c, const mcfl_sim* s, const double* Fx, uint64_t nper,
+                                             uint64_t seed, double* FlR, double* FlT, mcfl_fl_result* res) {
+  return fluorescence(c, s, Fx, nper, seed, nullptr, FlR, FlT, res);
+}
+
+extern "C" mcfl_status mcfl_run_fluorescence_lanes(mcfl_ctx* c, const mcfl_sim* s, const double* Fx, uint64_t nper,
+                                                   const mcfl_rng* rng, double* FlR, double* FlT, mcfl_fl_result* res) {
+  if (!rng || rng->mode != MCFL_RNG_LANES) return MCFL_ERR_ARG;
+  return fluorescence(c, s, Fx, nper, 0, rng, FlR, FlT, res);
 }

@@ -54,7 +54,7 @@ struct mcfl_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
-  int bpc[3][2][2][2] = {};  // resident workgroups per CU [work][temporal][perjob][layered], register-limited
+  int bpc[4][2][2][2] = {};  // resident workgroups per CU [work][temporal][perjob][layered], register-limited
   size_t lds_per_cu = 160 * 1024;
   uint32_t copies = MCFL_TALLY_COPIES;  // copies of the per-step volume tallies (1 or MCFL_TALLY_COPIES)
   int tile_side = 0;                    // lateral size of the fhd LDS tile, 0: from the LDS budget (env MCFL_TILE_SIDE)

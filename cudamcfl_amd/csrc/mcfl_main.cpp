@@ -3,8 +3,9 @@
 //
 // Same inputs (.mci, bulk file, safeprimes_base32.txt in the working
 // directory or the bundled table) and the same output files.  Extensions:
-//   -L        excitation on the reference's 32768 per-lane MWC streams
-//             (default: one stream per photon, full-GPU occupancy)
+//   -L        excitation and emission stages on the reference's 32768
+//             per-lane MWC streams, its launch replays included (default:
+//             one stream per photon, full-GPU occupancy)
 //   -F<seed>  seed of the emission stage (the reference uses time(NULL))
 //   -I<N>     emission stage by importance sampling: N photons launched from
 //             voxels drawn in proportion to eY (1 - albedof) Fx, an unbiased
@@ -75,7 +76,8 @@ std::vector<double> fx_of(const mcfl_sim* s, const std::vector<uint64_t>& fhd, u
 
 // PHD file and fluorescence stage after an excitation
 // (CUDAMCFLmain.cu:368-392,409-652).  Returns false if the emission failed.
-bool finish_file(mcfl_ctx* ctx, mcfl_sim* s, std::vector<double>& Fx, const Options& o, uint64_t* fl_photons) {
+bool finish_file(mcfl_ctx* ctx, mcfl_sim* s, std::vector<double>& Fx, const Options& o, const char* primes,
+                 uint64_t* fl_photons) {
   if (s->fhd_activated == 1) {
     printf("Writing PHD files...\n");
     mcfl_write_phd(s, Fx.data());
@@ -101,8 +103,23 @@ bool finish_file(mcfl_ctx* ctx, mcfl_sim* s, std::vector<double>& Fx, const Opti
   }
   mcfl_fl_result res;
   memset(&res, 0, sizeof res);
-  mcfl_status st =
-      mcfl_run_fluorescence(ctx, s, Fx.data(), s->number_of_photons_per_voxel, o.fl_seed, FlR.data(), FlT.data(), &res);
+  mcfl_status st;
+  if (o.lanes) {  // the reference's emission loop on its lanes, reseeded for this stage (CUDAMCFLmain.cu:412-413)
+    std::vector<uint64_t> x(MCFL_NUM_THREADS);
+    std::vector<uint32_t> a(MCFL_NUM_THREADS);
+    if (mcfl_init_rng(x.data(), a.data(), MCFL_NUM_THREADS, primes, o.fl_seed)) return false;
+    mcfl_rng rng;
+    memset(&rng, 0, sizeof rng);
+    rng.mode = MCFL_RNG_LANES;
+    rng.x = x.data();
+    rng.a = a.data();
+    rng.n_lanes = MCFL_NUM_THREADS;
+    st = mcfl_run_fluorescence_lanes(ctx, s, Fx.data(), s->number_of_photons_per_voxel, &rng, FlR.data(), FlT.data(),
+                                     &res);
+  } else {
+    st = mcfl_run_fluorescence(ctx, s, Fx.data(), s->number_of_photons_per_voxel, o.fl_seed, FlR.data(), FlT.data(),
+                               &res);
+  }
   if (st != MCFL_OK) {
     printf("Fluorescence failed: %s\n", mcfl_status_string(st));
     return false;
@@ -139,7 +156,7 @@ int run_one(mcfl_ctx* ctx, mcfl_sim* s, const char* primes, const Options& o, cl
     std::vector<double> Fx(nv, 0.0);
     fhd_photons = mcfl_DoOneSimulation(ctx, s, x.data(), a.data(), Fx.data());
     if (fhd_photons == 0) return 1;
-    const bool fl_ok = finish_file(ctx, s, Fx, o, &fl_photons);
+    const bool fl_ok = finish_file(ctx, s, Fx, o, primes, &fl_photons);
     report(s, fhd_photons, fl_photons, fl_ok, time0);
     return 0;
   }
@@ -165,7 +182,7 @@ int run_one(mcfl_ctx* ctx, mcfl_sim* s, const char* primes, const Options& o, cl
   mcfl_write_simulation_results(s, &t, secs);
   fhd_photons = t.num_terminated_photons;
   std::vector<double> Fx = fx_of(s, fhd, fhd_photons);
-  const bool fl_ok = finish_file(ctx, s, Fx, o, &fl_photons);
+  const bool fl_ok = finish_file(ctx, s, Fx, o, primes, &fl_photons);
   report(s, fhd_photons, fl_photons, fl_ok, time0);
   return 0;
 }
@@ -263,7 +280,7 @@ int run_many(mcfl_ctx* ctx, std::vector<mcfl_sim*>& sims, const std::vector<std:
       mcfl_write_simulation_results(s, &t, secs);
       uint64_t fl_photons = 0;
       std::vector<double> Fx = fx_of(s, fhd[j], t.num_terminated_photons);
-      const bool fl_ok = finish_file(ctx, s, Fx, o, &fl_photons);
+      const bool fl_ok = finish_file(ctx, s, Fx, o, primes, &fl_photons);
       report(s, t.num_terminated_photons, fl_photons, fl_ok, time0);
     }
   }

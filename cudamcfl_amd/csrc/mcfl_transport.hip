@@ -137,6 +137,14 @@ __device__ void launch(const MediumDev& m, int kind, float xi, float yi, float z
   p.tof = 0;
 }
 
+/* An emission photon from the corner of voxel v (CUDAMCFLmain.cu:481-487). */
+__device__ __forceinline__ void launch_voxel(const MediumDev& m, uint32_t v, Ph& p, uint64_t& x, uint32_t a) {
+  const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
+  const uint32_t iz = v / nxy, rem = v - iz * nxy, iy = rem / (uint32_t)m.nx, ix = rem - iy * (uint32_t)m.nx;
+  launch(m, MCFL_SRC_ISOTROPIC, ((float)ix / m.gs) - m.halfx, ((float)iy / m.gs) - m.halfy, ((float)iz / m.gs), 0.0f,
+         0u, p, x, a, false, 0u);
+}
+
 /* Spin, CUDAMCFLtransport.cu:585-625. */
 __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint32_t a) {
   float xi = mcfl_rand_co(&x, a);
@@ -393,6 +401,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   bool alive = false, retired = false;
   uint32_t lane_item = 0;
   uint64_t lane_left = 0;
+  uint32_t vj = 0; /* MCFL_WORK_VLANES: the lane's current voxel */
 
   /* the tile's job and origin as scalars (no LDS read on voxel changes) */
   const uint32_t tile_job = __builtin_amdgcn_readfirstlane(swg[1]);
@@ -407,7 +416,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   uint64_t pool_next = 0;
   bool pool_dead = false;
 
-  if (WORK == MCFL_WORK_VOXELS) {
+  if (WORK == MCFL_WORK_VOXELS || WORK == MCFL_WORK_VLANES) {
     mbase = A.jobs[0].mat_base;
     flags = A.jobs[0].flags;
     fhd = A.jobs[0].fhd ? A.jobs[0].fhd + xcc * A.jobs[0].copy_stride : nullptr;
@@ -499,6 +508,15 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           const float yi = ((float)iy / m.gs) - m.halfy;
           const float zi = ((float)iz / m.gs);
           launch(m, MCFL_SRC_ISOTROPIC, xi, yi, zi, 0.0f, 0u, p, x, a, false, 0u);
+        } else if (WORK == MCFL_WORK_VLANES) { /* the reference's emission loop on its lanes */
+          lane_item = (uint32_t)g;
+          x = A.xs[lane_item];
+          a = A.as[lane_item];
+          vj = 0;
+          lane_left = A.lane_q + ((uint64_t)lane_item < A.lane_rem ? 1ull : 0ull) - 1ull;
+          escale = A.vscale[0];
+          uint64_t xl = x; /* LaunchPhoton_Global does not store the advanced state */
+          launch_voxel(m, A.vox[0], p, xl, a);
         } else { /* reference lane streams */
           const JobDev& J = A.jobs[0];
           lane_item = (uint32_t)g;
@@ -693,6 +711,35 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
         } else {
           A.xs[lane_item] = x; /* CUDAMCFLmem.cu:42: advanced states go back */
         }
+      } else if (WORK == MCFL_WORK_VLANES) {
+        /* next photon of this voxel's simulation, or the next voxel's
+           simulation (a new LaunchPhoton_Global: the first photon replays) */
+        bool replay = false, more = true;
+        if (lane_left == 0ull) {
+          if (++vj < A.n_vox) {
+            lane_left = A.lane_q + ((uint64_t)lane_item < A.lane_rem ? 1ull : 0ull);
+            escale = A.vscale[vj];
+            replay = true;
+          } else {
+            more = false;
+          }
+        }
+        if (more) {
+          --lane_left;
+          accA = 0;
+          iters = 0;
+          if (replay) {
+            uint64_t xl = x;
+            launch_voxel(m, A.vox[vj], p, xl, a);
+          } else {
+            launch_voxel(m, A.vox[vj], p, x, a);
+          }
+          lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
+          lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
+          alive = true;
+        } else {
+          A.xs[lane_item] = x;
+        }
       }
     }
   }
@@ -775,6 +822,7 @@ const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t 
   }
   if (work == MCFL_WORK_VOXELS)
     return temporal ? kfn<MCFL_WORK_VOXELS, true, false, false>() : kfn<MCFL_WORK_VOXELS, false, false, false>();
+  if (work == MCFL_WORK_VLANES) return temporal ? nullptr : kfn<MCFL_WORK_VLANES, false, false, false>();
   return temporal ? kfn<MCFL_WORK_LANES, true, false, false>() : kfn<MCFL_WORK_LANES, false, false, false>();
 }
 }  // namespace
@@ -788,12 +836,13 @@ extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int 
 }
 
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream) {
-  if (A->work < 0 || A->work > 2 || nblocks <= 0) return -1;
+  if (A->work < 0 || A->work > 3 || nblocks <= 0) return -1;
   /* every launch reads the job table, the work counters and the multiplier
      table; the other pointers as its work kind needs them */
   if (!A->jobs || !A->counter || !A->a_tab || (A->n_mats && !A->mats) || A->njobs == 0 || A->njobs > MCFL_MAX_JOBS)
     return -1;
-  if (A->work == MCFL_WORK_LANES && (!A->xs || !A->as)) return -1;
+  if ((A->work == MCFL_WORK_LANES || A->work == MCFL_WORK_VLANES) && (!A->xs || !A->as)) return -1;
+  if (A->work == MCFL_WORK_VLANES && (!A->vox || !A->vscale || A->n_vox == 0 || A->lane_q == 0)) return -1;
   if (A->work == MCFL_WORK_VOXELS && (!A->vox || !A->vscale || A->per_job == 0)) return -1;
   if (!(A->work == MCFL_WORK_JOBS && perjob) && !A->med.labels && A->med.n_layers == 0) return -1;
   if (A->med.n_layers > 0 && !A->med.zb) return -1;

@@ -218,6 +218,21 @@ mcfl_status mcfl_fl_plan(const mcfl_sim* sim, const double* Fx, uint64_t n_per_v
 mcfl_status mcfl_run_fluorescence(mcfl_ctx* ctx, const mcfl_sim* sim, const double* Fx, uint64_t n_per_voxel,
                                   uint64_t seed, double* FlR, double* FlT, mcfl_fl_result* res);
 
+/* The same voxel loop on the reference's lane streams (DoOneSimulationFl per
+ * voxel, CUDAMCFLmain.cu:196-288 inside the loop :471-594): voxels in the
+ * reference's order (x outermost, z innermost), each one simulation of
+ * n_per_voxel photons on rng's MCFL_RNG_LANES lanes, which continue from voxel
+ * to voxel (rng->x advanced in place, CUDAMCFLmem.cu:42); in every voxel a
+ * lane's first photon replays its launch draws (CUDAMCFLtransport.cu:574-581).
+ * That replay correlates the first photon's launch with its first steps, and
+ * with n_per_voxel close to the lane count (as the reference's >= 32769
+ * photons on 32768 lanes) most emission photons are first photons: the images
+ * then carry the reference's own bias (percent-level), which the photon-stream
+ * loop above does not.  n_per_voxel < n_lanes is MCFL_ERR_PHOTONS (the
+ * reference's DoOneSimulationFl fails, CUDAMCFLmain.cu:268-272). */
+mcfl_status mcfl_run_fluorescence_lanes(mcfl_ctx* ctx, const mcfl_sim* sim, const double* Fx, uint64_t n_per_voxel,
+                                        const mcfl_rng* rng, double* FlR, double* FlT, mcfl_fl_result* res);
+
 /* ---- batched launches ----
  * No counterpart in the reference, whose multi-source scans run one process
  * per .mci file (ScriptGenerator-CeCAR-Translation.ipynb:48-55,

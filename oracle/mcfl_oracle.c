@@ -675,6 +675,52 @@ int orc_run_fluor_jobs(const orc_sim* sim, const uint32_t* a_tab, uint32_t P, ui
   return err;
 }
 
+/* The reference's emission loop on its lanes (CUDAMCFLmain.cu:471-594,
+ * DoOneSimulationFl :196-288) with the images in fixed point: voxels vox[j]
+ * in the reference's loop order, each one simulation of nv photons on the L
+ * lane streams that continue from voxel to voxel; in every simulation a
+ * lane's first photon replays its launch draws (LaunchPhoton_Global does not
+ * store the advanced state, CUDAMCFLtransport.cu:574-581) and the lane runs
+ * nv/L photons (the first nv%L lanes one more, as orc_run_lanes).  Exits add
+ * floor(w * scale[j] + 0.5).  x is advanced in place. */
+int orc_run_fluor_lanes(const orc_sim* sim, uint64_t* xs, const uint32_t* as, uint32_t L, uint32_t nvox,
+                        const uint32_t* vox, const double* scale, uint64_t nv, orc_tallies* out, int nthreads) {
+  if (nv < L) return -2; /* DoOneSimulationFl fails */
+  ctx_t* c = malloc(sizeof(ctx_t));
+  ctx_init(c, sim);
+  int nt = nthreads_eff(nthreads);
+  int err = 0;
+  const uint64_t q = nv / L, rem = nv % L;
+#pragma omp parallel num_threads(nt)
+  {
+    tl_t t;
+    if (!tl_alloc(&t, c, out, 1, 1.0)) err = 1;
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t i = 0; i < (int64_t)L; ++i) {
+      const uint64_t quota = q + ((uint64_t)i < rem ? 1u : 0u);
+      uint64_t x = xs[i];
+      const uint32_t a = as[i];
+      for (uint32_t j = 0; j < nvox; ++j) {
+        float xi, yi, zi;
+        voxel_corner(sim, vox[j], &xi, &yi, &zi);
+        t.exit_scale = scale[j];
+        orc_photon p;
+        uint64_t xl = x;
+        orc_launch(c, ORC_SRC_ISOTROPIC, xi, yi, zi, &p, &xl, a);
+        for (uint64_t k = 0; k < quota; ++k) {
+          if (k > 0) orc_launch(c, ORC_SRC_ISOTROPIC, xi, yi, zi, &p, &x, a);
+          run_photon(c, &p, &x, a, &t);
+        }
+      }
+      xs[i] = x;
+    }
+#pragma omp critical
+    tl_merge_free(&t, c, out);
+  }
+  free(c);
+  return err;
+}
+
 uint64_t orc_fluor_reference(const orc_sim* sim, const double* Fx, uint64_t* xs, const uint32_t* as, uint32_t L,
                              uint64_t nv, double* FlR, double* FlT, int nthreads) {
   ctx_t* c = malloc(sizeof(ctx_t));

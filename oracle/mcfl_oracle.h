@@ -21,6 +21,7 @@
  *   orc_run_lanes       CUDAMCFLmain.cu:60-165 + CUDAMCFLtransport.cu:565-582
  *                       (DoOneSimulation relaunch loop, per-lane streams)
  *   orc_fluor_reference CUDAMCFLmain.cu:409-594 (per-voxel emission loop)
+ *   orc_run_fluor_lanes CUDAMCFLmain.cu:196-288,471-594 (the same loop, fixed-point images)
  */
 #ifndef MCFL_ORACLE_H
 #define MCFL_ORACLE_H
@@ -127,6 +128,13 @@ int orc_run_lanes(const orc_sim* sim, uint64_t* x, const uint32_t* a, uint32_t L
 int orc_run_fluor_jobs(const orc_sim* sim_em, const uint32_t* a_tab, uint32_t P, uint32_t njobs,
                        const uint32_t* vox, const uint64_t* keys, const double* scale, uint64_t n_per_job,
                        orc_tallies* t, int nthreads);
+
+/* The reference's emission loop on its lane streams, images in fixed point:
+ * voxels vox[0..nvox) in the loop order, each one simulation of nv photons on
+ * the continuing lanes (first photon of each lane replays its launch draws);
+ * exits add floor(w*scale[j]+0.5).  x advanced in place.  -2 if nv < L. */
+int orc_run_fluor_lanes(const orc_sim* sim, uint64_t* x, const uint32_t* a, uint32_t L, uint32_t nvox,
+                        const uint32_t* vox, const double* scale, uint64_t nv, orc_tallies* t, int nthreads);
 
 /* Reference emission loop (CUDAMCFLmain.cu:471-594): every voxel whose
  * material has albedof < 1 runs one isotropic simulation of n_per_voxel

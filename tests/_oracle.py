@@ -64,6 +64,8 @@ def lib():
         L.orc_run_lanes.argtypes = [C.POINTER(Sim), U64P, U32P, C.c_uint32, C.c_uint64, C.POINTER(Tallies), C.c_int]
         L.orc_run_fluor_jobs.argtypes = [C.POINTER(Sim), U32P, C.c_uint32, C.c_uint32, U32P, U64P, F64P,
                                          C.c_uint64, C.POINTER(Tallies), C.c_int]
+        L.orc_run_fluor_lanes.argtypes = [C.POINTER(Sim), U64P, U32P, C.c_uint32, C.c_uint32, U32P, F64P,
+                                          C.c_uint64, C.POINTER(Tallies), C.c_int]
         L.orc_fluor_reference.restype = C.c_uint64
         L.orc_fluor_reference.argtypes = [C.POINTER(Sim), F64P, U64P, U32P, C.c_uint32, C.c_uint64, F64P, F64P,
                                           C.c_int]
@@ -237,6 +239,31 @@ def run_fluor_jobs(osim, a_tab, vox, keys, scale, n_per_job, nthreads=0):
     return r
 
 
+def fluor_reference(osim, Fx, x, a, n_per_voxel, nthreads=0):
+    """The reference's emission loop (every emitting voxel, n_per_voxel
+    photons on the continuing lane streams x, a; x advanced in place).
+    Returns (FlR, FlT, photons simulated)."""
+    xy = osim.det_nx * osim.det_ny
+    FlR, FlT = np.zeros(xy), np.zeros(xy)
+    Fx = np.ascontiguousarray(Fx, np.float64)
+    n = lib().orc_fluor_reference(C.byref(osim), ptr(Fx, F64P), ptr(x, U64P), ptr(a, U32P), len(a), n_per_voxel,
+                                  ptr(FlR, F64P), ptr(FlT, F64P), nthreads)
+    return FlR, FlT, int(n)
+
+
+def run_fluor_lanes(osim, x, a, vox, scale, n_per_voxel, nthreads=0):
+    """The reference's emission loop on its lanes, fixed-point images (x advanced in place)."""
+    r = Result(osim)
+    vox = np.ascontiguousarray(vox, np.uint32)
+    scale = np.ascontiguousarray(scale, np.float64)
+    rc = lib().orc_run_fluor_lanes(C.byref(osim), ptr(x, U64P), ptr(a, U32P), len(a), len(vox), ptr(vox, U32P),
+                                   ptr(scale, F64P), n_per_voxel, C.byref(r.t), nthreads)
+    if rc == -2:
+        raise ValueError("n_per_voxel < lanes")
+    assert rc == 0, rc
+    return r
+
+
 def _bind_extras(L):
     if getattr(L, "_extras", False):
         return
@@ -277,3 +304,13 @@ def tof_div_mismatches(lo, hi):
     L = lib()
     _bind_extras(L)
     return int(L.orc_tof_div_mismatches(lo, hi))
+
+
+def reference_order(sim, vox, scale):
+    """Emission voxels (x fastest, as fl_plan lists them) in the reference
+    loop's order: x outermost, then y, then z (CUDAMCFLmain.cu:471-473)."""
+    nx, ny = sim.s.nx, sim.s.ny
+    v = np.asarray(vox, np.int64)
+    ix, iy, iz = v % nx, (v // nx) % ny, v // (nx * ny)
+    order = np.lexsort((iz, iy, ix))
+    return np.asarray(vox)[order], np.asarray(scale)[order]

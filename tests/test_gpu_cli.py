@@ -103,3 +103,40 @@ def test_cli_importance_sampled_emission(tmp_path):
     # the excitation outputs do not depend on how the emission stage is run
     for suffix in ("_Refl.dat", "_Trans.dat", "_PHD-Ascii.dat"):
         assert (tmp_path / "loop" / ("f" + suffix)).read_bytes() == (tmp_path / "is" / ("f" + suffix)).read_bytes()
+
+
+def test_cli_lanes_both_stages(tmp_path, ctx, monkeypatch):
+    """`mcfl f.mci -L -S5 -F9` runs both stages on the reference's 32768
+    lane streams (excitation seeded with 5, emission reseeded with 9): its
+    fluorescence files equal those of mcfl_DoOneSimulation followed by
+    mcfl_run_fluorescence_lanes through the C ABI, written by the same writer."""
+    import numpy as np
+    from media import MATS3, small_labels
+    from cudamcfl_amd.media import write_bulk, write_mci
+    lab = small_labels(10, 10, 5, 3)
+    nz, ny, nx = lab.shape
+    write_bulk(tmp_path / "a.bulk", lab)
+    write_mci(tmp_path / "f.mci", "f", "a.bulk", MATS3, esp=nz / 10, grid_size=10, photons=40000,
+              photons_per_voxel=33000, do_fl=1, det=(0.1, 0.1, nx, ny, 0.0, 0.0), n_below=1.33, dims=(nx, ny, nz))
+    print("case: mcfl CLI -L, both stages on 32768 reference lanes", flush=True)
+    r = subprocess.run([str(EXE), "f.mci", "-L", "-S5", "-F9"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "All done, no errors" in r.stdout
+
+    api = tmp_path / "api"
+    api.mkdir()
+    monkeypatch.chdir(tmp_path)  # the bulk file is named relative to the working directory
+    sim = M.Simulation.from_mci("f.mci")
+    sim.s.outp_filename = str(api / "f").encode()
+    x, a = M.init_rng(M.NUM_THREADS, seed=5)
+    Fx = np.zeros(nx * ny * nz)
+    done = M.lib().mcfl_DoOneSimulation(ctx.h, C.byref(sim.s), x.ctypes.data_as(M.U64P), a.ctypes.data_as(M.U32P),
+                                        Fx.ctypes.data_as(M.F64P))
+    assert done == 40000
+    x2, a2 = M.init_rng(M.NUM_THREADS, seed=9)
+    fl = ctx.run_fluorescence_lanes(sim, Fx, 33000, lanes=(x2, a2))
+    assert fl["raw_R"].sum() > 0
+    M.lib().mcfl_write_fl(C.byref(sim.s), fl["FlR"].ctypes.data_as(M.F64P), fl["FlT"].ctypes.data_as(M.F64P))
+    for suffix in ("_FlR.dat", "_FlT.dat"):
+        assert (tmp_path / ("f" + suffix)).read_bytes() == (api / ("f" + suffix)).read_bytes(), suffix

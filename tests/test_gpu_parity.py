@@ -238,3 +238,32 @@ def test_reference_shaped_sphere_problem_bit_exact(ctx):
     re = O.run_fluor_jobs(oe.o, a_tab(), vox, keys, scale, 1)
     assert np.array_equal(fl["raw_R"], re.Rd) and np.array_equal(fl["raw_T"], re.Tt)
     check_totals(fl["totals"], re.tot)
+
+
+def test_fluorescence_on_reference_lanes_bit_exact(ctx):
+    """The emission loop on the reference's lane streams (voxels in the
+    reference's order, lanes continuing from voxel to voxel, first photon of
+    each lane replaying its launch draws): raw images, totals and the advanced
+    lane states bit-exact vs the oracle; fewer photons per voxel than lanes is
+    refused like the reference."""
+    lab = small_labels(10, 10, 5, 4)
+    sim = M.Simulation.voxel(lab, MATS3, esp=0.5, grid_size=10, det=(0.1, 0.1, 10, 10, 0.0, 0.0), photons=20000,
+                             n_above=1.0, n_below=1.33, fhd=True)
+    ex = ctx.run_excitation(sim, seed=11)
+    Fx = ex.fhd.astype(np.float64) / (0xFFFFFFFF * ex.totals["photons"])
+    L, nv = 512, 700
+    x0, a = M.init_rng(L, seed=17)
+    print("case: fluorescence on 512 reference lanes, 700 photons per voxel", flush=True)
+    xg = x0.copy()
+    fl = ctx.run_fluorescence_lanes(sim, Fx, nv, lanes=(xg, a))
+    vox, scale, unit = M.fl_plan(sim, Fx, nv)
+    vox, scale = O.reference_order(sim, vox, scale)
+    xo = x0.copy()
+    o = O.FromProduct(sim, emission=True)
+    r = O.run_fluor_lanes(o.o, xo, a, vox, scale, nv)
+    assert np.array_equal(fl["raw_R"], r.Rd) and np.array_equal(fl["raw_T"], r.Tt)
+    check_totals(fl["totals"], r.tot)
+    assert np.array_equal(xg, xo) and not np.array_equal(xg, x0)
+    assert fl["raw_R"].sum() > 0 and fl["unit"] == unit
+    with pytest.raises(M.McflError):
+        ctx.run_fluorescence_lanes(sim, Fx, L - 1, lanes=(x0.copy(), a))

@@ -469,3 +469,68 @@ def test_layered_sample_mci_energy_identity(a_tab):
     assert t["launched_w"] + t["roul_gain"] - t["roul_loss"] == t["R"] + t["T"] + t["A"]
     assert 0 < t["R"] and 0 < t["A"]
     assert r.fhd.sum() > 0
+
+
+def test_fluorescence_estimators_agree(a_tab):
+    """The reference's emission loop (one simulation per emitting voxel on
+    lane streams that continue from voxel to voxel, CUDAMCFLmain.cu:471-594)
+    and this build's voxel emission jobs on photon streams estimate the same
+    FlR and FlT images: totals agree within Monte Carlo noise over 4 seeds
+    each."""
+    import media
+    lab = media.small_labels(10, 10, 5, 4)
+    sim = M.Simulation.voxel(lab, media.MATS3, esp=0.5, grid_size=10, det=(0.1, 0.1, 10, 10, 0.0, 0.0),
+                             photons=20000, n_above=1.0, n_below=1.33, fhd=True)
+    o = O.FromProduct(sim)
+    ex = O.run_photons(o.o, a_tab, M.stream_key(3, 0), 0, 20000)
+    Fx = ex.fhd.astype(np.float64) / (4294967295.0 * 20000)
+    oe = O.FromProduct(sim, emission=True)
+    nv, L = 130, 64
+    xy = 100
+    dxdy = float(np.float32(0.1)) * float(np.float32(0.1))
+    vox, scale, unit = M.fl_plan(sim, Fx, nv)
+    ref, ps = [], []
+    for seed in range(4):
+        x, a = M.init_rng(L, seed=seed + 1)
+        FlR, FlT, n = O.fluor_reference(oe.o, Fx, x, a, nv)
+        assert n > 0
+        ref.append((FlR.sum(), FlT.sum()))
+        keys = np.array([M.stream_key(seed + 10, int(v)) for v in vox], np.uint64)
+        r = O.run_fluor_jobs(oe.o, a_tab, vox, keys, scale, nv)
+        ps.append((r.Rd[:xy].sum() * unit / dxdy, r.Tt[:xy].sum() * unit / dxdy))
+    ref, ps = np.array(ref), np.array(ps)
+    for k in range(2):
+        m1, m2 = ref[:, k].mean(), ps[:, k].mean()
+        sig = math.sqrt(ref[:, k].var(ddof=1) / 4 + ps[:, k].var(ddof=1) / 4)
+        assert m1 > 0 and abs(m1 - m2) < 5 * sig + 1e-3 * m1, (k, m1, m2, sig)
+
+
+def test_fixed_point_lane_emission_equals_reference_loop(a_tab):
+    """The lane-stream emission loop with fixed-point images (what the GPU runs)
+    reproduces the reference's loop with its double accumulation
+    (CUDAMCFLmain.cu:552-577): same advanced lane states, images equal to
+    1e-9 relative."""
+    import media
+    lab = media.small_labels(10, 10, 5, 4)
+    sim = M.Simulation.voxel(lab, media.MATS3, esp=0.5, grid_size=10, det=(0.1, 0.1, 10, 10, 0.0, 0.0),
+                             photons=20000, n_above=1.0, n_below=1.33, fhd=True)
+    o = O.FromProduct(sim)
+    ex = O.run_photons(o.o, a_tab, M.stream_key(3, 0), 0, 20000)
+    Fx = ex.fhd.astype(np.float64) / (4294967295.0 * 20000)
+    oe = O.FromProduct(sim, emission=True)
+    nv, L = 100, 64
+    x0, a = M.init_rng(L, seed=5)
+    x1 = x0.copy()
+    FlR, FlT, _ = O.fluor_reference(oe.o, Fx, x1, a, nv)
+    vox, scale, unit = M.fl_plan(sim, Fx, nv)
+    vox, scale = O.reference_order(sim, vox, scale)
+    x2 = x0.copy()
+    r = O.run_fluor_lanes(oe.o, x2, a, vox, scale, nv)
+    assert np.array_equal(x1, x2)
+    dxdy = float(np.float32(0.1)) * float(np.float32(0.1))
+    for img, raw in ((FlR, r.Rd), (FlT, r.Tt)):
+        fx = raw[:100].astype(np.float64) * unit / dxdy
+        assert img.sum() > 0
+        np.testing.assert_allclose(fx, img, rtol=1e-9, atol=1e-12 * img.max())
+    with pytest.raises(ValueError):
+        O.run_fluor_lanes(oe.o, x0.copy(), a, vox, scale, L - 1)
