@@ -396,6 +396,13 @@ def main():
     steps_per_s = steps / dev_s
     achieved = steps_per_s * F_STEP / 1e12
     peak = PEAK_SLOTS / 1e12
+    traffic = measured_traffic(N)
+    ms_step = sum(kms) / args.steps
+    # the secondary roofline: HBM bytes per launch over the launch time, against 8 TB/s
+    hbm = None
+    if traffic:
+        gbs = traffic / (ms_step * 1e-3) / 1e9
+        hbm = {"achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0}
     out = {
         "metric": "photons/s at 128x128x64 voxels",
         "value": value,
@@ -403,7 +410,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sum(kms) / args.steps,
+        "ms_per_step": ms_step,
         "wall_ms_per_step": 1e3 * sum(walls) / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -417,7 +424,7 @@ def main():
         "photon_steps_per_s": steps_per_s,
         "steps_per_photon": steps / (N * args.steps),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tslot/s",
-                     "frac": achieved / peak, "traffic": measured_traffic(N),
+                     "frac": achieved / peak, "traffic": traffic, "hbm": hbm,
                      "note": f"{F_STEP} VALU issue slots per photon-step x photon-steps / kernel time; traffic = "
                              "HBM bytes per launch from FETCH_SIZE/WRITE_SIZE scaled on a copy (profiles/)"},
         "gpu": info,
