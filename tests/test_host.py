@@ -265,3 +265,45 @@ def test_reference_shaped_sphere_problem():
     assert abs(b[1].mua - 0.11) < 1e-6 and abs(b[2].mua - 0.6) < 1e-6        # mua + flc AbsCExi
     assert abs(b[1].muaf - 0.08) < 1e-6 and abs(b[2].muaf - 0.08) < 1e-6      # mua + flc AbsCEmi
     assert abs(b[1].mutr - 1.0 / 100.1) < 1e-7 and abs(b[2].mutr - 1.0 / 100.5) < 1e-7
+
+
+def test_entry_points_refuse_null_arguments():
+    """Every entry point that takes a context, batch or server handle returns
+    MCFL_ERR_ARG (or 0 photons) for a null handle or a null required pointer,
+    before touching the device."""
+    lib = M.lib()
+    M._bind_engine(lib)
+    ERR_ARG = 1
+    sim = M.Simulation.voxel(np.ones((2, 2, 2), np.uint8), [(1.4, 1.0, 100.0, 0.0, 0.0, 0.0, 0.1, 0.9)],
+                             esp=0.2, grid_size=10, det=(0.1, 0.1, 4, 4, 0.0, 0.0), photons=10)
+    S = C.byref(sim.s)
+    Fx = np.zeros(8)
+    F = Fx.ctypes.data_as(M.F64P)
+    rng = M.Rng()
+    tune = M.Tune()
+    calls = {
+        "mcfl_ctx_create": lambda: lib.mcfl_ctx_create(0, None, None),
+        "mcfl_ctx_info": lambda: lib.mcfl_ctx_info(None, None, None, None, 0),
+        "mcfl_run_excitation": lambda: lib.mcfl_run_excitation(None, S, C.byref(rng), None),
+        "mcfl_run_fluorescence": lambda: lib.mcfl_run_fluorescence(None, S, F, 10, 1, F, F, None),
+        "mcfl_run_fluorescence_lanes": lambda: lib.mcfl_run_fluorescence_lanes(None, S, F, 10, None, F, F, None),
+        "mcfl_fl_plan": lambda: lib.mcfl_fl_plan(None, F, 10, None, None, None, None),
+        "mcfl_batch_create": lambda: lib.mcfl_batch_create(None, None, 0, None, 0, None),
+        "mcfl_batch_run": lambda: lib.mcfl_batch_run(None, None, None),
+        "mcfl_batch_autotune": lambda: lib.mcfl_batch_autotune(None, 0.1, None),
+        "mcfl_batch_configure": lambda: lib.mcfl_batch_configure(None, C.byref(tune)),
+        "mcfl_run_batch": lambda: lib.mcfl_run_batch(None, None, 0, None, 0, None),
+        "mcfl_server_create": lambda: lib.mcfl_server_create(None, 0, 1, None),
+        "mcfl_server_add_medium": lambda: lib.mcfl_server_add_medium(None, S, None),
+        "mcfl_server_submit": lambda: lib.mcfl_server_submit(None, None, None),
+        "mcfl_server_step": lambda: lib.mcfl_server_step(None, None, 0, None, None),
+        "mcfl_server_collect": lambda: lib.mcfl_server_collect(None, 0, None),
+        "mcfl_server_stats": lambda: lib.mcfl_server_stats(None, None, None, None),
+        "mcfl_server_counters": lambda: lib.mcfl_server_counters(None, None, None, None, None),
+    }
+    for name, call in calls.items():
+        assert call() == ERR_ARG, name
+    assert lib.mcfl_DoOneSimulation(None, S, None, None, F) == 0
+    lib.mcfl_ctx_destroy(None)
+    lib.mcfl_batch_destroy(None)
+    lib.mcfl_server_destroy(None)
