@@ -178,4 +178,14 @@ MCFL_HD double mcfl_tof_div(double q) {
   return mcfl_fmad(mcfl_fmad(-y, MCFL_CCMFS, q), r, y);
 }
 
+/* The time-of-flight increment of a float q = step * n: the quotient q / c
+   truncated to uint64 with mcfl_d2u64_rz's rules.  Below 2^32 fs (steps
+   shorter than 1.29e5 cm) that is one double -> uint32 conversion; NaN and
+   larger values take the general conversion. */
+MCFL_HD uint64_t mcfl_tof_inc(float q) {
+  const double t = mcfl_tof_div((double)q);
+  if (t < 4294967296.0) return t > 0.0 ? (uint64_t)(uint32_t)t : 0ull;
+  return mcfl_d2u64_rz(t);
+}
+
 #endif /* MCFL_FMATH_H */

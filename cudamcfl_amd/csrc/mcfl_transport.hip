@@ -235,7 +235,7 @@ __device__ uint32_t glass_march(const MediumDev& m, Ph& p, uint32_t old_lab, flo
     p.x = mcfl_fma(p.dx, step, p.x);
     p.y = mcfl_fma(p.dy, step, p.y);
     p.z = mcfl_fma(p.dz, step, p.z);
-    if (TEMPORAL) p.tof += mcfl_d2u64_rz(mcfl_tof_div((double)step));
+    if (TEMPORAL) p.tof += mcfl_tof_inc(step);
     if (mcfl_fma(p.dz, step, p.z) < 0.0f) {
       present = 0;
       index = -1;
@@ -280,7 +280,7 @@ __device__ uint32_t glass_march(const MediumDev& m, Ph& p, uint32_t old_lab, flo
     p.x = first ? x1 : x2;
     p.y = first ? y1 : y2;
     p.z = first ? z1 : z2;
-    if (TEMPORAL) p.tof -= mcfl_d2u64_rz(mcfl_tof_div((double)(first ? t1 : t2)));
+    if (TEMPORAL) p.tof -= mcfl_tof_inc(first ? t1 : t2);
   }
   return present;
 }
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       p.x = mcfl_fma(p.dx, s, p.x);
       p.y = mcfl_fma(p.dy, s, p.y);
       p.z = mcfl_fma(p.dz, s, p.z);
-      if (TEMPORAL) p.tof += mcfl_d2u64_rz(mcfl_tof_div((double)(s * ml.n)));
+      if (TEMPORAL) p.tof += mcfl_tof_inc(s * ml.n);
       if (p.z > zhi) p.z = zhi; /* :125-126 */
       if (p.z < zlo) p.z = zlo;
       in = in_grid(m, p.x, p.y, p.z);
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
         p.x = mcfl_fma(p.dx, s, p.x);
         p.y = mcfl_fma(p.dy, s, p.y);
         p.z = mcfl_fma(p.dz, s, p.z);
-        if (TEMPORAL) p.tof += mcfl_d2u64_rz(mcfl_tof_div((double)(s * mc.n)));
+        if (TEMPORAL) p.tof += mcfl_tof_inc(s * mc.n);
       } else {
         s = 100.0f;
         nb = glass_march<TEMPORAL>(m, p, p.lab, s);

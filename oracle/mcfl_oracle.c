@@ -867,7 +867,7 @@ uint64_t orc_tof_div_mismatches(uint32_t lo, uint32_t hi) {
   for (int64_t b = (int64_t)lo; b < (int64_t)hi; ++b) {
     const float f = mcfl_bitsf((uint32_t)b);
     const double q = (double)f;
-    if (!(mcfl_tof_div(q) == q / C_CMFS)) ++bad;
+    if (!(mcfl_tof_div(q) == q / C_CMFS) || mcfl_tof_inc(f) != mcfl_d2u64_rz(q / C_CMFS)) ++bad;
   }
   return bad;
 }

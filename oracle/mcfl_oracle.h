@@ -163,7 +163,9 @@ void orc_math_eval(int fn, const float* in, float* out, size_t n);
 /* n draws of the [0,1) generator from state x: states after each step and the draws */
 void orc_mwc_draws(uint64_t x, uint32_t a, size_t n, uint64_t* states, float* co);
 uint64_t orc_photon_state(uint64_t key, uint64_t k, uint32_t a);
-/* float bit patterns in [lo, hi) where mcfl_tof_div(q) != q / 2.99792e-5 */
+/* float bit patterns in [lo, hi) where mcfl_tof_div(q) != q / 2.99792e-5, or
+   where the kernel's increment mcfl_tof_inc(q) differs from the quotient
+   truncated by mcfl_d2u64_rz */
 uint64_t orc_tof_div_mismatches(uint32_t lo, uint32_t hi);
 
 #ifdef __cplusplus

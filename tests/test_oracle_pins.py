@@ -87,7 +87,9 @@ def test_tof_quotient_is_the_ieee_division():
     """The kernel's time-of-flight increment q / 2.99792e-5 (a product and one
     FMA correction) equals the IEEE double quotient of the reference's
     expression (CUDAMCFLtransport.cu:316) for every non-negative finite float q,
-    and for a stretch of negative ones (the formula is odd)."""
+    and for a stretch of negative ones (the formula is odd); the kernel's
+    increment (a 32-bit conversion below 2^32) truncates it like the general
+    conversion for all of them."""
     assert O.tof_div_mismatches(0, 0x7F800000) == 0
     assert O.tof_div_mismatches(0x80000000, 0x80000000 + (1 << 26)) == 0
     assert O.tof_div_mismatches(0xC0000000, 0xC0000000 + (1 << 26)) == 0
