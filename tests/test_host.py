@@ -307,3 +307,22 @@ def test_entry_points_refuse_null_arguments():
     lib.mcfl_ctx_destroy(None)
     lib.mcfl_batch_destroy(None)
     lib.mcfl_server_destroy(None)
+
+
+def test_integration_example_compiles_and_links(tmp_path):
+    """The C snippet that shows a maintainer how to replace the reference's
+    main() compiles as C against include/mcfl.h and links against libmcfl.so."""
+    import shutil
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    code = re.search(r"```c\n(.*?)```", text, re.S).group(1)
+    body = "\n".join("  " + ln for ln in code.splitlines() if not ln.startswith("#include"))
+    src = ("#include <stdlib.h>\n#include <time.h>\n#include \"mcfl.h\"\n"
+           "int main(int argc, char** argv) {\n  uint64_t seed = 1;\n" + body + "\n  (void)done;\n  return 0;\n}\n")
+    (tmp_path / "main.c").write_text(src)
+    build = os.path.dirname(M.LIB_PATH)
+    r = subprocess.run(["gcc", "-O2", "-Wall", "-Werror", "-o", str(tmp_path / "cuda_fl"), str(tmp_path / "main.c"),
+                        "-I" + os.path.join(ROOT, "include"), "-L" + build, "-lmcfl", "-Wl,-rpath," + build],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
