@@ -443,6 +443,7 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   if (const char* e = getenv("MCFL_TILE_SIDE")) c->tile_side = atoi(e);
   if (const char* e = getenv("MCFL_REFILL_MIN")) c->refill_min = (uint32_t)std::min(64, std::max(1, atoi(e)));
   if (const char* e = getenv("MCFL_CHUNK_MAX")) c->chunk_max = (uint32_t)std::min(1 << 20, std::max(64, atoi(e)));
+  if (const char* e = getenv("MCFL_RUN_PHOTONS")) c->run_photons = std::max(1ull, strtoull(e, nullptr, 10));
   snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
   for (int w = 0; w < 4; ++w)
     for (int t = 0; t < 2; ++t)
@@ -468,8 +469,9 @@ extern "C" void mcfl_ctx_destroy(mcfl_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->labels, &c->mats, &c->jobs, &c->counter, &c->tot, &c->Rd, &c->Tt, &c->fhd, &c->axyz, &c->time,
-                    &c->Rd2, &c->Tt2, &c->tpos, &c->xs, &c->as, &c->vox, &c->vscale})
+  for (DevBuf* b : {&c->labels, &c->zb, &c->mats, &c->jobs, &c->counter, &c->tot, &c->Rd, &c->Tt, &c->fhd, &c->axyz,
+                    &c->time, &c->Rd2, &c->Tt2, &c->tpos, &c->xs, &c->as, &c->vox, &c->vscale, &c->fhd_acc,
+                    &c->axyz_acc})
     release(*b);
   if (c->d_atab) (void)hipFree(c->d_atab);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -601,14 +603,63 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     choose_tile(c, A, A.med, temporal, 0);
     place_tile(A, A.med, s->xi, s->yi, J);
   }
-  HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
-  auto reduce = [&]() -> int {
-    int e = 0;
-    if (fhd_on) e |= mcfl_reduce_copies(c->fhd.as<unsigned long long>(), nv, copies, c->stream);
-    if (out->A_xyz) e |= mcfl_reduce_copies(c->axyz.as<unsigned long long>(), nv, copies, c->stream);
-    return e;
-  };
-  STCHK(launch_and_time(c, A, temporal, 0, &out->kernel_ms, reduce));
+  const uint64_t N = s->number_of_photons;
+  if (lanes || N <= c->run_photons) {
+    HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
+    auto reduce = [&]() -> int {
+      int e = 0;
+      if (fhd_on) e |= mcfl_reduce_copies(c->fhd.as<unsigned long long>(), nv, copies, c->stream);
+      if (out->A_xyz) e |= mcfl_reduce_copies(c->axyz.as<unsigned long long>(), nv, copies, c->stream);
+      return e;
+    };
+    STCHK(launch_and_time(c, A, temporal, 0, &out->kernel_ms, reduce));
+    HIPCHK(hipMemcpy(out->totals, c->tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost));
+  } else {
+    /* Photons [k0, k0 + n) per launch: photon k draws from stream k whatever
+       launch runs it, so the tallies equal one launch's, except that a
+       voxel's fhd / A_xyz and the weight totals saturate at 2^64-1 instead of
+       wrapping (each launch's copies stay far below the u64 range). */
+    if (fhd_on) {
+      STCHK(ensure(c->fhd_acc, nv * 8));
+      HIPCHK(hipMemsetAsync(c->fhd_acc.p, 0, nv * 8, c->stream));
+    }
+    if (out->A_xyz) {
+      STCHK(ensure(c->axyz_acc, nv * 8));
+      HIPCHK(hipMemsetAsync(c->axyz_acc.p, 0, nv * 8, c->stream));
+    }
+    memset(out->totals, 0, sizeof out->totals);
+    out->kernel_ms = 0.0;
+    for (uint64_t k0 = 0; k0 < N; k0 += c->run_photons) {
+      const uint64_t n = std::min(c->run_photons, N - k0);
+      J.k0 = k0;
+      J.count = n;
+      A.total = n;
+      HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(c->counter.p, 0, 8, c->stream));
+      HIPCHK(hipMemsetAsync(c->tot.p, 0, MCFL_NTOT * 8, c->stream));
+      auto accumulate = [&]() -> int {
+        int e = 0;
+        if (fhd_on)
+          e |= mcfl_accumulate_copies(c->fhd.as<unsigned long long>(), nv, copies,
+                                      c->fhd_acc.as<unsigned long long>(), c->stream);
+        if (out->A_xyz)
+          e |= mcfl_accumulate_copies(c->axyz.as<unsigned long long>(), nv, copies,
+                                      c->axyz_acc.as<unsigned long long>(), c->stream);
+        return e;
+      };
+      double ms = 0.0;
+      STCHK(launch_and_time(c, A, temporal, 0, &ms, accumulate));
+      out->kernel_ms += ms;
+      uint64_t t[MCFL_NTOT];
+      HIPCHK(hipMemcpy(t, c->tot.p, sizeof t, hipMemcpyDeviceToHost));
+      for (int i = 0; i < MCFL_NTOT; ++i) {
+        const uint64_t a = out->totals[i] + t[i];
+        out->totals[i] = a < t[i] ? ~0ull : a;
+      }
+    }
+    if (fhd_on) HIPCHK(hipMemcpyAsync(c->fhd.p, c->fhd_acc.p, nv * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (out->A_xyz) HIPCHK(hipMemcpyAsync(c->axyz.p, c->axyz_acc.p, nv * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
 
   HIPCHK(hipMemcpy(out->Rd_xy, c->Rd.p, npix * 8, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(out->Tt_xy, c->Tt.p, npix * 8, hipMemcpyDeviceToHost));
@@ -619,7 +670,6 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     HIPCHK(hipMemcpy(out->Tt2_xy, c->Tt2.p, npix * 8, hipMemcpyDeviceToHost));
   }
   if (temporal) HIPCHK(hipMemcpy(out->time_xyt, c->time.p, ntime * 8, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(out->totals, c->tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost));
   if (lanes) HIPCHK(hipMemcpy(rng->x, c->xs.p, (size_t)rng->n_lanes * 8, hipMemcpyDeviceToHost));
   out->num_terminated_photons = out->totals[MCFL_TOT_PHOTONS];
   return MCFL_OK;

@@ -18,6 +18,10 @@ extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int 
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream);
 extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
 extern "C" int mcfl_reduce_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, hipStream_t stream);
+// acc[i] += the sum of the ncopies copies of element i, both saturating at
+// 2^64-1; the copies are zeroed (ready for the next launch of a chunked run).
+extern "C" int mcfl_accumulate_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, unsigned long long* acc,
+                                      hipStream_t stream);
 
 namespace mcfl_detail {
 
@@ -60,10 +64,12 @@ struct mcfl_ctx {
   int tile_side = 0;                    // lateral size of the fhd LDS tile, 0: from the LDS budget (env MCFL_TILE_SIDE)
   uint32_t refill_min = 4;              // idle lanes that trigger a wave's refill (env MCFL_REFILL_MIN)
   uint32_t chunk_max = 1024;            // largest photon chunk a wave takes from the counter (env MCFL_CHUNK_MAX)
+  uint64_t run_photons = 1ull << 27;    // photon-stream excitations above this run as several launches
+                                        // of at most this many photons (env MCFL_RUN_PHOTONS)
   char name[256] = {};
   uint32_t* d_atab = nullptr;
   std::vector<uint32_t> atab;
-  mcfl_detail::DevBuf labels, zb, mats, jobs, counter, tot, Rd, Tt, fhd, axyz, time, Rd2, Tt2, tpos, xs, as, vox, vscale;
+  mcfl_detail::DevBuf labels, zb, mats, jobs, counter, tot, Rd, Tt, fhd, axyz, time, Rd2, Tt2, tpos, xs, as, vox, vscale, fhd_acc, axyz_acc;
 };
 
 namespace mcfl_detail {

@@ -771,11 +771,32 @@ __global__ void mcfl_fill_u64_kernel(unsigned long long* p, uint64_t n) {
 }
 
 /* Sums copies 1..ncopies-1 of a per-XCD volume tally into copy 0. */
+/* Sum of the ncopies copies of element i, saturated at 2^64-1 (a voxel whose
+   total weight exceeds the u64 range keeps the largest value instead of
+   wrapping; the reference stops adding at LLONG_MAX, CUDAMCFLtransport.cu:358-366). */
+__device__ __forceinline__ unsigned long long sat_sum_copies(const unsigned long long* p, uint64_t n, uint32_t ncopies,
+                                                             uint64_t i) {
+  unsigned long long s = p[i];
+  bool over = false;
+  for (uint32_t c = 1; c < ncopies; ++c) {
+    const unsigned long long v = p[(uint64_t)c * n + i];
+    s += v;
+    over |= s < v;
+  }
+  return over ? ~0ull : s;
+}
+
 __global__ void mcfl_reduce_copies_kernel(unsigned long long* p, uint64_t n, uint32_t ncopies) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = sat_sum_copies(p, n, ncopies, i);
+}
+
+__global__ void mcfl_accumulate_copies_kernel(unsigned long long* p, uint64_t n, uint32_t ncopies,
+                                              unsigned long long* acc) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    unsigned long long s = p[i];
-    for (uint32_t c = 1; c < ncopies; ++c) s += p[(uint64_t)c * n + i];
-    p[i] = s;
+    const unsigned long long s = sat_sum_copies(p, n, ncopies, i), a = acc[i] + s;
+    acc[i] = a < s ? ~0ull : a;
+    for (uint32_t c = 0; c < ncopies; ++c) p[(uint64_t)c * n + i] = 0ull;
   }
 }
 
@@ -864,6 +885,16 @@ extern "C" int mcfl_reduce_copies(unsigned long long* p, uint64_t n, uint32_t nc
   uint64_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   mcfl_reduce_copies_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(p, n, ncopies);
+  return (int)hipGetLastError();
+}
+
+extern "C" int mcfl_accumulate_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, unsigned long long* acc,
+                                      hipStream_t stream) {
+  if (n == 0) return 0;
+  if (!p || !acc || ncopies == 0) return -1;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  mcfl_accumulate_copies_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(p, n, ncopies, acc);
   return (int)hipGetLastError();
 }
 

@@ -181,7 +181,16 @@ typedef struct mcfl_tallies {
 /* The device part of DoOneSimulation (CUDAMCFLmain.cu:60-167): runs one
  * excitation simulation to completion in a single persistent launch and
  * returns the integer tallies.  Outputs are overwritten.  number_of_photons
- * = 0 is MCFL_ERR_ARG (the reference's parser never produces it). */
+ * = 0 is MCFL_ERR_ARG (the reference's parser never produces it).
+ * A voxel of fhd / A_xyz whose summed weight exceeds 2^64 - 1 returns
+ * 2^64 - 1 (the reference stops adding at LLONG_MAX, CUDAMCFLtransport.cu:364).
+ * On photon streams a run above 2^27 photons (env MCFL_RUN_PHOTONS at context
+ * creation) is split into launches of photons [k0, k0 + n) with the same
+ * results; the weight totals then saturate the same way.  Each launch keeps
+ * its volume tallies in 8 per-XCD u64 copies: a launch of n photons is exact
+ * while n * F < 8 * 2^32, F being the full weights one photon deposits on
+ * average in the hottest voxel (6 for the reference's sample2.mci), which
+ * the 2^27-photon split guarantees for F < 256. */
 mcfl_status mcfl_run_excitation(mcfl_ctx* ctx, const mcfl_sim* sim, const mcfl_rng* rng, mcfl_tallies* out);
 
 /* DoOneSimulation, CUDAMCFLmain.cu:60-193: reference lane streams (x, a of

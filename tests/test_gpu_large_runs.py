@@ -1,0 +1,72 @@
+"""Runs whose volume tallies approach the u64 range.  A plain photon-stream
+excitation above a photon threshold runs as several launches of photons
+[k0, k0 + n); photon k always draws from stream k, so the tallies equal those
+of one launch.  A voxel whose summed weight exceeds 2^64 - 1 keeps 2^64 - 1
+(the reference stops adding at LLONG_MAX, CUDAMCFLtransport.cu:364) instead of
+wrapping, in one launch and across launches."""
+import numpy as np
+import pytest
+
+import cudamcfl_amd as M
+import _oracle as O
+from media import small_sim
+
+pytestmark = pytest.mark.gpu
+
+U64_MAX = np.uint64(2 ** 64 - 1)
+
+
+def _same(a, b):
+    for n in ("Rd", "Tt", "fhd", "A_xyz", "time", "Rd2", "Tt2"):
+        x, y = getattr(a, n, None), getattr(b, n, None)
+        if x is None and y is None:
+            continue
+        assert np.array_equal(x, y), n
+    assert a.totals == b.totals
+
+
+def test_split_run_equals_one_launch(ctx, monkeypatch):
+    """250000 photons as launches of 100000 + 100000 + 50000 (temporal
+    detectors, fhd, A_xyz, squared weights): every tally equal to one launch's
+    and to the oracle's."""
+    sim = small_sim(photons=250000)
+    monkeypatch.setenv("MCFL_RUN_PHOTONS", "100000")
+    split = M.Context(0)
+    print("case: 20x20x10, 250000 photons, one launch vs three", flush=True)
+    a = ctx.run_excitation(sim, seed=9, axyz=True, sq=True)
+    b = split.run_excitation(sim, seed=9, axyz=True, sq=True)
+    _same(a, b)
+    assert b.totals["photons"] == 250000 and b.fhd.sum() > 0 and b.time.sum() > 0
+    o = O.FromProduct(sim, axyz=True, sq=True)
+    r = O.run_photons(o.o, M.multipliers(32768), M.stream_key(9, 0), 0, 250000)
+    assert np.array_equal(b.fhd, r.fhd) and np.array_equal(b.Rd, r.Rd)
+
+
+def _hot_voxel_sim(photons):
+    """A 3x3x1 grid of 1 cm voxels, isotropic scattering (mus 50/cm, g 0):
+    each photon puts ~119 full weights of fhd into the source voxel, so 1e8
+    photons need 2.8 times the u64 range there (and ~0.35 times in each of the
+    8 per-XCD copies)."""
+    lab = np.ones((1, 3, 3), np.uint8)
+    return M.Simulation.voxel(lab, [(1.4, 0.01, 50.0, 0.0, 0.0, 0.0, 0.1, 0.0)], esp=1.0, grid_size=1,
+                              det=(1.0, 1.0, 3, 3, 0.0, 0.0), photons=photons, n_above=1.0, n_below=1.0, fhd=True)
+
+
+def test_hot_voxel_saturates_instead_of_wrapping(ctx, monkeypatch):
+    N = 100_000_000
+    small = ctx.run_excitation(_hot_voxel_sim(N // 8), seed=4)
+    centre = int(np.argmax(small.fhd))
+    assert float(small.fhd[centre]) * 8 > 2.0 ** 64 * 1.5  # the full run exceeds the u64 range there
+    sim = _hot_voxel_sim(N)
+    print("case: 3x3x1 hot voxel, 1e8 photons, one launch and four", flush=True)
+    a = ctx.run_excitation(sim, seed=4, axyz=True)
+    monkeypatch.setenv("MCFL_RUN_PHOTONS", "30000000")
+    b = M.Context(0).run_excitation(sim, seed=4, axyz=True)
+    for r in (a, b):
+        assert r.totals["photons"] == N
+        assert r.fhd[centre] == U64_MAX
+        others = np.delete(r.fhd, centre)
+        assert others.max() < U64_MAX and others.sum() > 0
+    assert np.array_equal(np.delete(a.fhd, centre), np.delete(b.fhd, centre))
+    assert np.array_equal(a.A_xyz, b.A_xyz) and np.array_equal(a.Rd, b.Rd) and np.array_equal(a.Tt, b.Tt)
+    assert a.totals == b.totals
