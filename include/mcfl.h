@@ -248,7 +248,12 @@ mcfl_status mcfl_run_fluorescence_lanes(mcfl_ctx* ctx, const mcfl_sim* sim, cons
  * cudamcfl-array.sh:15).  Many simulations -- different sources, budgets and
  * media -- share one persistent launch and one photon queue; each job's
  * photons draw from streams keyed by its own seed, so every job's tallies are
- * identical to a run of that job alone. */
+ * identical to a run of that job alone.  A job's fhd / A_xyz are summed from
+ * per-XCD u64 copies and saturate at 2^64 - 1; they are exact while
+ * photons * F < 8 * 2^32 (F as for mcfl_run_excitation), so a job far above
+ * 2^27 photons in a strongly scattering hot voxel belongs in
+ * mcfl_run_excitation, which splits it.  Served requests (below) tally fhd
+ * only internally, for their emission pass, with the same bound. */
 
 enum { MCFL_TALLY_FHD = 1, MCFL_TALLY_SQ = 2, MCFL_TALLY_TEMP = 4, MCFL_TALLY_AXYZ = 8 };
 #define MCFL_MAX_BATCH_JOBS 64
