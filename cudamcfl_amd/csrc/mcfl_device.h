@@ -161,6 +161,7 @@ struct LaunchArgs {
   uint32_t chunk_max;
   uint32_t refill_min; /* a wave refills when this many of its lanes are idle (1..64) */
   uint32_t wg;         /* threads per workgroup: 0 (MCFL_WG), or MCFL_WG_ALT1/2 for job launches */
+  uint32_t max_steps;  /* per-photon step cap; 0: MCFL_MAX_STEPS_PER_PHOTON */
 };
 
 /* Dynamic LDS of a launch: per-job totals, fhd tile, material table, layer bounds. */

@@ -394,6 +394,7 @@ struct Launcher {
     A->njobs = (uint32_t)jd.size();
     A->chunk_max = c->chunk_max;
     A->refill_min = c->refill_min;
+    A->max_steps = c->max_steps;
     return MCFL_OK;
   }
   // Job table upload, after the tiles are placed.

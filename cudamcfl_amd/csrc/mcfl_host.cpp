@@ -289,6 +289,7 @@ mcfl_status launch_and_time(mcfl_ctx* c, LaunchArgs& A, int temporal, int perjob
   A.n_waves = (uint32_t)nblocks * (mcfl_wg_of(&A) / 64u);
   if (A.chunk_max == 0) A.chunk_max = c->chunk_max;
   A.refill_min = c->refill_min;
+  A.max_steps = c->max_steps;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (mcfl_transport_launch(&A, temporal, perjob, nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
   if (post && post() != 0) return MCFL_ERR_DEVICE;
@@ -444,6 +445,7 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   if (const char* e = getenv("MCFL_REFILL_MIN")) c->refill_min = (uint32_t)std::min(64, std::max(1, atoi(e)));
   if (const char* e = getenv("MCFL_CHUNK_MAX")) c->chunk_max = (uint32_t)std::min(1 << 20, std::max(64, atoi(e)));
   if (const char* e = getenv("MCFL_RUN_PHOTONS")) c->run_photons = std::max(1ull, strtoull(e, nullptr, 10));
+  if (const char* e = getenv("MCFL_MAX_STEPS")) c->max_steps = (uint32_t)std::min(1ull << 31, strtoull(e, nullptr, 10));
   snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
   for (int w = 0; w < 4; ++w)
     for (int t = 0; t < 2; ++t)

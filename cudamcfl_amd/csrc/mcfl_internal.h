@@ -64,6 +64,7 @@ struct mcfl_ctx {
   int tile_side = 0;                    // lateral size of the fhd LDS tile, 0: from the LDS budget (env MCFL_TILE_SIDE)
   uint32_t refill_min = 4;              // idle lanes that trigger a wave's refill (env MCFL_REFILL_MIN)
   uint32_t chunk_max = 1024;            // largest photon chunk a wave takes from the counter (env MCFL_CHUNK_MAX)
+  uint32_t max_steps = 0;               // per-photon step cap, 0: MCFL_MAX_STEPS_PER_PHOTON (env MCFL_MAX_STEPS)
   uint64_t run_photons = 1ull << 27;    // photon-stream excitations above this run as several launches
                                         // of at most this many photons (env MCFL_RUN_PHOTONS)
   char name[256] = {};

@@ -402,6 +402,9 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   uint32_t lane_item = 0;
   uint64_t lane_left = 0;
   uint32_t vj = 0; /* MCFL_WORK_VLANES: the lane's current voxel */
+  /* bounded work per photon: a photon still alive after this many steps is
+     stopped and its weight tallied as aborted */
+  const uint32_t max_steps = A.max_steps ? A.max_steps : MCFL_MAX_STEPS_PER_PHOTON;
 
   /* the tile's job and origin as scalars (no LDS read on voxel changes) */
   const uint32_t tile_job = __builtin_amdgcn_readfirstlane(swg[1]);
@@ -679,7 +682,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       p.w = 0u;
       lives = false;
     }
-    if (lives && iters >= MCFL_MAX_STEPS_PER_PHOTON) {
+    if (lives && iters >= max_steps) {
       lds_add_u64(&st[MCFL_DT_ABORTED], p.w);
       p.w = 0u;
       lives = false;

@@ -162,7 +162,8 @@ enum {
   MCFL_TOT_ZERO_KILL = 6,  /* weight dropped when the absorbed quantum rounds to 0 */
   MCFL_TOT_PHOTONS = 7,    /* photons launched */
   MCFL_TOT_STEPS = 8,      /* photon steps (transport loop iterations) */
-  MCFL_TOT_ABORTED = 9     /* weight of photons stopped by the per-photon step cap */
+  MCFL_TOT_ABORTED = 9     /* weight of photons stopped by the per-photon step cap (2^28 steps;
+                              env MCFL_MAX_STEPS at context creation lowers it) */
 };
 
 typedef struct mcfl_tallies {

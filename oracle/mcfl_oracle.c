@@ -16,6 +16,9 @@
 
 #include "../cudamcfl_amd/csrc/mcfl_mwc.h"
 
+static uint32_t g_max_steps = ORC_MAX_STEPS_PER_PHOTON;
+void orc_set_max_steps(uint32_t n) { g_max_steps = n ? n : ORC_MAX_STEPS_PER_PHOTON; }
+
 #define WEIGHTI 429497u     /* CUDAMCFL.h:33 */
 #define CHANCE 0.1f         /* CUDAMCFL.h:34 */
 #define C_CMFS 2.99792E-5   /* CUDAMCFL.h:29, cm per fs */
@@ -518,7 +521,7 @@ static void run_photon(const ctx_t* c, orc_photon* p, uint64_t* x, uint32_t a, t
   t->tot[ORC_T_LAUNCHED_W] += p->weight;
   uint32_t n = 0;
   while (orc_step(c, p, x, a, t)) {
-    if (++n >= ORC_MAX_STEPS_PER_PHOTON) {
+    if (++n >= g_max_steps) {
       t->tot[ORC_T_ABORTED] += p->weight;
       p->weight = 0u;
       break;

@@ -105,6 +105,9 @@ typedef struct {
 
 /* Per-photon step cap of the transport loop (bounded-work watchdog). */
 #define ORC_MAX_STEPS_PER_PHOTON (1u << 28)
+/* Sets the cap for later runs (0: ORC_MAX_STEPS_PER_PHOTON); not thread-safe
+   against runs in progress. */
+void orc_set_max_steps(uint32_t n);
 
 /* init_RNG restatement: reads n_rng+1 lines of the primes file.  0 = OK. */
 int orc_init_rng(uint64_t* x, uint32_t* a, uint32_t n_rng, const char* primes_file, uint64_t xinit);

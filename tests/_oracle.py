@@ -273,6 +273,7 @@ def _bind_extras(L):
     L.orc_photon_state.restype = C.c_uint64
     L.orc_tof_div_mismatches.argtypes = [C.c_uint32, C.c_uint32]
     L.orc_tof_div_mismatches.restype = C.c_uint64
+    L.orc_set_max_steps.argtypes = [C.c_uint32]
     L._extras = True
 
 
@@ -298,6 +299,13 @@ def photon_state(key, k, a):
     L = lib()
     _bind_extras(L)
     return int(L.orc_photon_state(key, k, a))
+
+
+def set_max_steps(n):
+    """The oracle's per-photon step cap for later runs (0: the default)."""
+    L = lib()
+    _bind_extras(L)
+    L.orc_set_max_steps(int(n))
 
 
 def tof_div_mismatches(lo, hi):

@@ -267,3 +267,26 @@ def test_fluorescence_on_reference_lanes_bit_exact(ctx):
     assert fl["raw_R"].sum() > 0 and fl["unit"] == unit
     with pytest.raises(M.McflError):
         ctx.run_fluorescence_lanes(sim, Fx, L - 1, lanes=(x0.copy(), a))
+
+
+def test_step_cap_aborts_like_the_oracle(ctx, monkeypatch):
+    """The per-photon step cap (the kernel's bounded-work watchdog) forced
+    down to 40 steps from the host: photons still alive after 40 steps stop
+    and their weight is tallied as aborted; tallies, totals and the energy
+    balance equal the oracle run with the same cap."""
+    monkeypatch.setenv("MCFL_MAX_STEPS", "40")
+    capped = M.Context(0)
+    sim = small_sim(photons=20000)
+    print("case: photon streams, 20x20x10, step cap 40", flush=True)
+    g = capped.run_excitation(sim, seed=31, axyz=True, sq=True)
+    O.set_max_steps(40)
+    try:
+        o = O.FromProduct(sim, axyz=True, sq=True)
+        r = O.run_photons(o.o, a_tab(), M.stream_key(31, 0), 0, 20000)
+    finally:
+        O.set_max_steps(0)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert g.totals["aborted"] > 0 and balance(g.totals) == 0
+    assert g.totals["steps"] <= 40 * 20000
+    capped.close()

@@ -361,6 +361,28 @@ def test_energy_balance_and_counts(a_tab):
     assert int(r.Rd.sum()) <= t["R"] and int(r.Tt.sum()) <= t["T"]
 
 
+def test_step_cap_keeps_the_energy_balance(a_tab):
+    """With the per-photon step cap at 40 (default 2^28), photons alive after
+    their 40th step stop, their weight goes to the aborted total, no photon
+    takes more than 40 steps, and the balance still closes; with the cap at 39
+    more weight is aborted."""
+    import media
+    sim = media.small_sim(photons=3000)
+    o = O.FromProduct(sim)
+    res = {}
+    try:
+        for cap in (40, 39):
+            O.set_max_steps(cap)
+            res[cap] = O.run_photons(o.o, a_tab, 5, 0, 3000).tot
+    finally:
+        O.set_max_steps(0)
+    for cap, t in res.items():
+        assert t["aborted"] > 0 and t["steps"] <= cap * 3000
+        assert t["launched_w"] + t["roul_gain"] - t["roul_loss"] - t["zero_kill"] - t["aborted"] == \
+            t["R"] + t["T"] + t["A"]
+    assert res[39]["aborted"] > res[40]["aborted"]
+
+
 def test_thread_count_does_not_change_tallies(a_tab):
     import media
     sim = media.small_sim(photons=2000)
