@@ -3,7 +3,8 @@ excitation above a photon threshold runs as several launches of photons
 [k0, k0 + n); photon k always draws from stream k, so the tallies equal those
 of one launch.  A voxel whose summed weight exceeds 2^64 - 1 keeps 2^64 - 1
 (the reference stops adding at LLONG_MAX, CUDAMCFLtransport.cu:364) instead of
-wrapping, in one launch and across launches."""
+wrapping, in one launch and across launches.  Contexts used from several host
+threads at once give the results of the same runs made alone."""
 import numpy as np
 import pytest
 
@@ -70,3 +71,33 @@ def test_hot_voxel_saturates_instead_of_wrapping(ctx, monkeypatch):
     assert np.array_equal(np.delete(a.fhd, centre), np.delete(b.fhd, centre))
     assert np.array_equal(a.A_xyz, b.A_xyz) and np.array_equal(a.Rd, b.Rd) and np.array_equal(a.Tt, b.Tt)
     assert a.totals == b.totals
+
+
+def test_contexts_on_two_host_threads(ctx):
+    """The ABI is thread-compatible per context: two host threads, each with
+    its own context (and stream), run excitations at the same time; each
+    result equals the same run made alone."""
+    import threading
+    sims = [small_sim(photons=400000), _hot_voxel_sim(400000)]
+    alone = [ctx.run_excitation(s, seed=21 + i, axyz=True) for i, s in enumerate(sims)]
+    ctxs = [M.Context(0), M.Context(0)]
+    out, errs = [None, None], []
+
+    def work(i):
+        try:
+            for _ in range(3):
+                out[i] = ctxs[i].run_excitation(sims[i], seed=21 + i, axyz=True)
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    print("case: two contexts on two host threads", flush=True)
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errs and all(not t.is_alive() for t in th)
+    for a, b in zip(alone, out):
+        _same(a, b)
+    for c in ctxs:
+        c.close()
