@@ -217,7 +217,8 @@ class Simulation:
             b.muaf = f32(mua + f32(flc * emi))
             b.eY, b.flc, b.g = ey, flc, g
             b.mutr = np.finfo(f32).max if mus == 0 else f32(f32(1) / f32(mua + mus))
-            b.albedof = f32(mus / f32(f32(flc * exi) + mus))
+            with np.errstate(invalid="ignore"):  # 0/0 = NaN for a glass without fluorophore, as the parser gives
+                b.albedof = f32(mus / f32(f32(flc * exi) + mus))
         obj._keep += [lab, B]
         s.number_of_photons = photons
         s.number_of_photons_per_voxel = photons_per_voxel
@@ -281,7 +282,8 @@ class Simulation:
             dtot = f32(dtot + d)
             L.z_max = dtot
             L.mutr = np.finfo(f32).max if mus == 0 else f32(f32(1) / f32(mua + mus))
-            L.albedof = f32(mus / f32(f32(flc * exi) + mus))
+            with np.errstate(invalid="ignore"):  # 0/0 = NaN, as the parser gives
+                L.albedof = f32(mus / f32(f32(flc * exi) + mus))
         x, y, z, r, n, mua, mus, flc, exi, emi, ey, g = [f32(v) for v in inclusion]
         inc = s.inclusion
         inc.x, inc.y, inc.z, inc.r, inc.n = x, y, z, r, n
