@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -326,3 +327,11 @@ def test_integration_example_compiles_and_links(tmp_path):
                         "-I" + os.path.join(ROOT, "include"), "-L" + build, "-lmcfl", "-Wl,-rpath," + build],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N above 1 exits with a message before
+    touching a device."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU" in (r.stdout + r.stderr)
