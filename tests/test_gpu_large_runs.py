@@ -101,3 +101,26 @@ def test_contexts_on_two_host_threads(ctx):
         _same(a, b)
     for c in ctxs:
         c.close()
+
+
+def test_largest_grid_bit_exact():
+    """A 1024x1024x512 grid (2^29 voxels: 512 MiB of labels, 4 GiB of fhd per
+    copy, 32 GiB for the 8 per-XCD copies on the device): voxel indices, copy
+    offsets and the copy reduction stay exact; fhd, images and totals equal
+    the oracle's (one oracle thread, so it holds a single extra fhd array)."""
+    from media import MATS3
+    nz, ny, nx = 512, 1024, 1024
+    lab = np.ones((nz, ny, nx), np.uint8)
+    lab[20:60, 480:560, 480:560] = 2
+    sim = M.Simulation.voxel(lab, MATS3, esp=nz / 20, grid_size=20, det=(0.05, 0.05, 256, 256, 0.0, 0.0),
+                             photons=20000, fhd=True)
+    del lab
+    print("case: 1024x1024x512 grid, 20000 photons", flush=True)
+    big = M.Context(0)
+    g = big.run_excitation(sim, seed=13)
+    big.close()
+    o = O.FromProduct(sim)
+    r = O.run_photons(o.o, M.multipliers(32768), M.stream_key(13, 0), 0, 20000, nthreads=1)
+    assert g.totals == r.tot
+    assert np.array_equal(g.Rd, r.Rd) and np.array_equal(g.Tt, r.Tt)
+    assert g.fhd.sum() > 0 and np.array_equal(g.fhd, r.fhd)
