@@ -140,3 +140,30 @@ def test_cli_lanes_both_stages(tmp_path, ctx, monkeypatch):
     M.lib().mcfl_write_fl(C.byref(sim.s), fl["FlR"].ctypes.data_as(M.F64P), fl["FlT"].ctypes.data_as(M.F64P))
     for suffix in ("_FlR.dat", "_FlT.dat"):
         assert (tmp_path / ("f" + suffix)).read_bytes() == (api / ("f" + suffix)).read_bytes(), suffix
+
+
+def test_cli_lanes_on_the_layered_sample(tmp_path):
+    """`mcfl sample.mci -L -S7` (method 1, the reference's 32768 lanes): the
+    run finishes every photon and writes the reference's files; its image
+    files equal those of mcfl_DoOneSimulation through the C ABI with the same
+    seed (which writes them itself)."""
+    text = (GOLDEN / "sample.mci").read_text()
+    (tmp_path / "cli").mkdir()
+    (tmp_path / "cli" / "sample.mci").write_text(text.replace("500000000", "60000", 1))
+    print("case: mcfl CLI -L, sample.mci with 60000 photons", flush=True)
+    r = subprocess.run([str(EXE), "sample.mci", "-L", "-S7"], cwd=tmp_path / "cli", capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "All done, no errors" in r.stdout
+    api = tmp_path / "api"
+    api.mkdir()
+    sim = M.Simulation.from_mci(str(tmp_path / "cli" / "sample.mci"))
+    sim.s.outp_filename = str(api / "sample1-out").encode()
+    x, a = M.init_rng(M.NUM_THREADS, seed=7)
+    ctx = M.Context(0)
+    done = M.lib().mcfl_DoOneSimulation(ctx.h, C.byref(sim.s), x.ctypes.data_as(M.U64P), a.ctypes.data_as(M.U32P),
+                                        None)
+    ctx.close()
+    assert done == 60000
+    for suffix in ("_Refl.dat", "_Trans.dat"):
+        assert (tmp_path / "cli" / ("sample1-out" + suffix)).read_bytes() == (api / ("sample1-out" + suffix)).read_bytes()
