@@ -68,7 +68,8 @@ class Sim(C.Structure):
 
 
 class Rng(C.Structure):
-    _fields_ = [("mode", C.c_int32), ("seed", C.c_uint64), ("x", U64P), ("a", U32P), ("n_lanes", C.c_uint32)]
+    _fields_ = [("mode", C.c_int32), ("seed", C.c_uint64), ("x", U64P), ("a", U32P), ("n_lanes", C.c_uint32),
+                ("first_photon", C.c_uint64)]
 
 
 class Tallies(C.Structure):
@@ -398,8 +399,9 @@ class Context:
         check(lib().mcfl_ctx_info(self.h, C.byref(ncu), C.byref(bpc), name, 256), "mcfl_ctx_info")
         return {"n_cu": ncu.value, "blocks_per_cu": bpc.value, "arch": name.value.decode()}
 
-    def run_excitation(self, sim, seed=None, lanes=None, axyz=False, sq=False):
-        """mcfl_run_excitation.  seed -> photon streams; lanes=(x, a) -> reference lanes (x advanced)."""
+    def run_excitation(self, sim, seed=None, lanes=None, axyz=False, sq=False, first_photon=0):
+        """mcfl_run_excitation.  seed -> photon streams (photons first_photon onwards);
+        lanes=(x, a) -> reference lanes (x advanced)."""
         r = ExcitationResult(sim, axyz=axyz, sq=sq)
         rng = Rng()
         if lanes is not None:
@@ -407,6 +409,7 @@ class Context:
             rng.mode, rng.x, rng.a, rng.n_lanes = RNG_LANES, _p(x, U64P), _p(a, U32P), len(a)
         else:
             rng.mode, rng.seed = RNG_PHOTON, int(seed if seed is not None else 1)
+            rng.first_photon = int(first_photon)
         check(lib().mcfl_run_excitation(self.h, C.byref(sim.s), C.byref(rng), C.byref(r.t)), "run_excitation")
         return r
 

@@ -498,6 +498,7 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
   if (lanes) {
     if (!rng->x || !rng->a || rng->n_lanes == 0) return MCFL_ERR_ARG;
     if (s->number_of_photons < rng->n_lanes) return MCFL_ERR_PHOTONS;  // the reference never terminates
+    if (rng->first_photon != 0) return MCFL_ERR_ARG;
   } else if (rng->mode != MCFL_RNG_PHOTON) {
     return MCFL_ERR_ARG;
   }
@@ -528,6 +529,7 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
   memset(&J, 0, sizeof J);
   J.begin = 0;
   J.count = lanes ? rng->n_lanes : s->number_of_photons;
+  J.k0 = lanes ? 0 : rng->first_photon;
   J.key = ::mcfl_key(rng->seed, 0);
   J.src_kind = src_kind_of(s);
   J.xi = s->xi;
@@ -633,7 +635,7 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     out->kernel_ms = 0.0;
     for (uint64_t k0 = 0; k0 < N; k0 += c->run_photons) {
       const uint64_t n = std::min(c->run_photons, N - k0);
-      J.k0 = k0;
+      J.k0 = rng->first_photon + k0;
       J.count = n;
       A.total = n;
       HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));

@@ -150,6 +150,12 @@ typedef struct mcfl_rng {
   uint64_t* x;       /* MCFL_RNG_LANES: per-lane states (advanced in place, CUDAMCFLmem.cu:42) */
   const uint32_t* a; /* MCFL_RNG_LANES: per-lane multipliers */
   uint32_t n_lanes;  /* MCFL_RNG_LANES: number of lanes (32768 in the reference) */
+  /* MCFL_RNG_PHOTON (extension): the run simulates photons [first_photon,
+   * first_photon + number_of_photons) of the seed's streams, 0 for a whole
+   * run.  Tallies of disjoint ranges add up to those of their union: a run
+   * can be split across processes or GPUs, or resumed, and summed exactly.
+   * Must be 0 for MCFL_RNG_LANES. */
+  uint64_t first_photon;
 } mcfl_rng;
 
 enum {

@@ -5,6 +5,8 @@ of one launch.  A voxel whose summed weight exceeds 2^64 - 1 keeps 2^64 - 1
 (the reference stops adding at LLONG_MAX, CUDAMCFLtransport.cu:364) instead of
 wrapping, in one launch and across launches.  Contexts used from several host
 threads at once give the results of the same runs made alone."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -124,3 +126,31 @@ def test_largest_grid_bit_exact():
     assert g.totals == r.tot
     assert np.array_equal(g.Rd, r.Rd) and np.array_equal(g.Tt, r.Tt)
     assert g.fhd.sum() > 0 and np.array_equal(g.fhd, r.fhd)
+
+
+def test_photon_ranges_add_up(ctx, monkeypatch):
+    """Photons [0, 100000) and [100000, 300000) of a seed, run separately (the
+    second also split into launches), add up to the run of [0, 300000): the
+    primitive for splitting a run across GPUs or resuming it.  A photon
+    offset with the reference's lane streams is refused."""
+    N = 300_000
+    full = ctx.run_excitation(small_sim(photons=N), seed=19, axyz=True, sq=True)
+    a = ctx.run_excitation(small_sim(photons=100_000), seed=19, axyz=True, sq=True)
+    monkeypatch.setenv("MCFL_RUN_PHOTONS", "70000")
+    split = M.Context(0)
+    print("case: photon ranges [0,1e5) + [1e5,3e5) vs [0,3e5)", flush=True)
+    b = split.run_excitation(small_sim(photons=200_000), seed=19, axyz=True, sq=True, first_photon=100_000)
+    split.close()
+    for n in ("Rd", "Tt", "fhd", "A_xyz", "time", "Rd2", "Tt2"):
+        assert np.array_equal(getattr(a, n) + getattr(b, n), getattr(full, n)), n
+    for k in full.totals:
+        assert a.totals[k] + b.totals[k] == full.totals[k], k
+    x, am = M.init_rng(512, seed=3)
+    rng = M.Rng()
+    rng.mode, rng.x, rng.a, rng.n_lanes, rng.first_photon = M.RNG_LANES, x.ctypes.data_as(M.U64P), \
+        am.ctypes.data_as(M.U32P), 512, 5
+    t = M.Tallies()
+    Rd = np.zeros(small_sim().npix, np.uint64)
+    t.Rd_xy, t.Tt_xy = Rd.ctypes.data_as(M.U64P), Rd.ctypes.data_as(M.U64P)
+    sim = small_sim(photons=5000)
+    assert M.lib().mcfl_run_excitation(ctx.h, C.byref(sim.s), C.byref(rng), C.byref(t)) == 1  # MCFL_ERR_ARG

@@ -335,3 +335,31 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 2 and "one GPU" in (r.stdout + r.stderr)
+
+
+def test_python_mirror_matches_the_c_layout(tmp_path):
+    """Every ctypes structure of the Python binding has the size and field
+    offsets of its C counterpart in include/mcfl.h (compiled here with gcc)."""
+    import shutil
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    pairs = [("mcfl_layer", M.Layer), ("mcfl_bulk", M.Bulk), ("mcfl_inclusion", M.Inclusion), ("mcfl_det", M.Det),
+             ("mcfl_sim", M.Sim), ("mcfl_rng", M.Rng), ("mcfl_tallies", M.Tallies), ("mcfl_fl_result", M.FlResult),
+             ("mcfl_job", M.Job), ("mcfl_request", M.Request), ("mcfl_request_result", M.RequestResult),
+             ("mcfl_tune", M.Tune)]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "mcfl.h"', "int main(void) {"]
+    expect = []
+    for cname, py in pairs:
+        lines.append(f'  printf("%zu\\n", sizeof({cname}));')
+        expect.append(C.sizeof(py))
+        for f in py._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({cname}, {f[0]}));')
+            expect.append(getattr(py, f[0]).offset)
+    lines += ["  return 0;", "}"]
+    (tmp_path / "layout.c").write_text("\n".join(lines) + "\n")
+    exe = str(tmp_path / "layout")
+    r = subprocess.run(["gcc", "-o", exe, str(tmp_path / "layout.c"), "-I" + os.path.join(ROOT, "include")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == expect
