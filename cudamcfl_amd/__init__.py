@@ -409,7 +409,7 @@ class Context:
             rng.mode, rng.x, rng.a, rng.n_lanes = RNG_LANES, _p(x, U64P), _p(a, U32P), len(a)
         else:
             rng.mode, rng.seed = RNG_PHOTON, int(seed if seed is not None else 1)
-            rng.first_photon = int(first_photon)
+        rng.first_photon = int(first_photon)  # the library refuses an offset on lane streams
         check(lib().mcfl_run_excitation(self.h, C.byref(sim.s), C.byref(rng), C.byref(r.t)), "run_excitation")
         return r
 

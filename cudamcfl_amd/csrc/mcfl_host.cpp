@@ -441,6 +441,10 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   c->n_cu = prop.multiProcessorCount;
   if (const char* e = getenv("MCFL_TALLY_COPIES")) c->copies = atoi(e) > 1 ? MCFL_TALLY_COPIES : 1;
+  // the default split keeps one launch's share of a voxel below the u64 range
+  // of one copy for F < 256 full weights per photon: with a single copy the
+  // launches are 8 times shorter
+  if (c->copies == 1) c->run_photons /= MCFL_TALLY_COPIES;
   if (const char* e = getenv("MCFL_TILE_SIDE")) c->tile_side = atoi(e);
   if (const char* e = getenv("MCFL_REFILL_MIN")) c->refill_min = (uint32_t)std::min(64, std::max(1, atoi(e)));
   if (const char* e = getenv("MCFL_CHUNK_MAX")) c->chunk_max = (uint32_t)std::min(1 << 20, std::max(64, atoi(e)));
@@ -664,6 +668,10 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     if (fhd_on) HIPCHK(hipMemcpyAsync(c->fhd.p, c->fhd_acc.p, nv * 8, hipMemcpyDeviceToDevice, c->stream));
     if (out->A_xyz) HIPCHK(hipMemcpyAsync(c->axyz.p, c->axyz_acc.p, nv * 8, hipMemcpyDeviceToDevice, c->stream));
   }
+  // The copies below run on the null stream, which does not wait for the
+  // context's non-blocking stream: finish its work (the accumulated copies
+  // above, or the clears of a skipped launch) first.
+  HIPCHK(hipStreamSynchronize(c->stream));
 
   HIPCHK(hipMemcpy(out->Rd_xy, c->Rd.p, npix * 8, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(out->Tt_xy, c->Tt.p, npix * 8, hipMemcpyDeviceToHost));
@@ -756,6 +764,7 @@ static mcfl_status fluorescence(mcfl_ctx* c, const mcfl_sim* s, const double* Fx
   const bool lanes = rng != nullptr;
   if (lanes) {
     if (!rng->x || !rng->a || rng->n_lanes == 0) return MCFL_ERR_ARG;
+    if (rng->first_photon != 0) return MCFL_ERR_ARG;  // photon offsets exist for photon streams only
     if (nper < rng->n_lanes) return MCFL_ERR_PHOTONS;  // DoOneSimulationFl fails (CUDAMCFLmain.cu:268-272)
   }
   if (hipSetDevice(c->device) != hipSuccess) return MCFL_ERR_DEVICE;
@@ -863,6 +872,7 @@ static mcfl_status fluorescence(mcfl_ctx* c, const mcfl_sim* s, const double* Fx
   A.vseed = seed;
   double ms = 0.0;
   if (A.total > 0) STCHK(launch_and_time(c, A, 0, 0, &ms));
+  HIPCHK(hipStreamSynchronize(c->stream));  // the clears above when no launch ran
 
   std::vector<uint64_t> R(npix), T(npix);
   HIPCHK(hipMemcpy(R.data(), c->Rd.p, npix * 8, hipMemcpyDeviceToHost));

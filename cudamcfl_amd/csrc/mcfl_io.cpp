@@ -420,7 +420,7 @@ extern "C" int mcfl_write_simulation_results(const mcfl_sim* sim, const mcfl_tal
     return 1;
   }
   fprintf(pf, "Compilation date: %s, %s. \n\n", __DATE__, __TIME__);
-  fprintf(pf, "A1   # Version number of the file format.\n\n");
+  fprintf(pf, "A1 \t# Version number of the file format.\n\n");
   fprintf(pf, "####\n");
   fprintf(pf, "# Data categories include: \n");
   fprintf(pf, "# InParm, RAT, \n");

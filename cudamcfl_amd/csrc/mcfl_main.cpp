@@ -313,6 +313,10 @@ int main(int argc, char* argv[]) {
   std::vector<char*> args{argv[0]};
   int i = 1;
   for (; i < argc && argv[i][0] != '-'; ++i) files.push_back(argv[i]);
+  if (files.empty()) {  // the reference takes argv[1] as the input file whatever it is (CUDAMCFLmain.cu:305-310)
+    files.push_back(argv[1]);
+    i = 2;
+  }
   args.push_back(argv[1]);
   for (; i < argc; ++i) {
     if (!strcmp(argv[i], "-L")) {

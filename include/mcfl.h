@@ -197,7 +197,10 @@ typedef struct mcfl_tallies {
  * its volume tallies in 8 per-XCD u64 copies: a launch of n photons is exact
  * while n * F < 8 * 2^32, F being the full weights one photon deposits on
  * average in the hottest voxel (6 for the reference's sample2.mci), which
- * the 2^27-photon split guarantees for F < 256. */
+ * the 2^27-photon split guarantees for F < 256.  With MCFL_TALLY_COPIES=1 in
+ * the environment (one copy) the split is 2^24 photons, for the same bound.
+ * Runs on MCFL_RNG_LANES are one launch whatever N: they are exact while
+ * N * F < 8 * 2^32 (N * F < 2^32 with one copy). */
 mcfl_status mcfl_run_excitation(mcfl_ctx* ctx, const mcfl_sim* sim, const mcfl_rng* rng, mcfl_tallies* out);
 
 /* DoOneSimulation, CUDAMCFLmain.cu:60-193: reference lane streams (x, a of

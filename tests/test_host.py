@@ -169,7 +169,13 @@ def test_writers_byte_layout(tmp_path, monkeypatch):
     assert t.startswith("#x y timebins[numbins]\n")
     assert "%f %f " % (-0.5, 0.0) + " %E " % (5 / scale) + " %E " % (7 / scale) + " \n " in t
     p = open(tmp_path / "w_param.dat", "rb").read()
-    assert b"# User time: 1.25 sec" in p
+    # the header of CUDAMCFLio.cu:100-112 byte for byte (after the build-date line)
+    head = (b"A1 \t# Version number of the file format.\n\n####\n# Data categories include: \n"
+            b"# InParm, RAT, \n# A_l, A_z, Rd_r, Rd_a, Tt_r, Tt_a, \n# A_rz, Rd_ra, Tt_ra \n####\n\n"
+            b"# User time: 1.25 sec\n\nInParam\t\t# Input parameters:\n")
+    first, rest = p.split(b"\n\n", 1)
+    assert first.startswith(b"Compilation date: ") and first.endswith(b". ")
+    assert rest.startswith(head)
     raw = open("sample2.mci", "rb").read()
     assert p.split(b"InParam\t\t# Input parameters:\n", 1)[1] == raw[s.begin:s.end]
 
@@ -210,6 +216,16 @@ def test_cli_without_arguments():
     exe = os.path.join(M.BUILD_DIR, "mcfl")
     p = subprocess.run([exe], capture_output=True, text=True)
     assert p.returncode == 1 and "Not enough input arguments!" in p.stdout
+
+
+def test_cli_flag_in_place_of_the_input_file():
+    """A flag where the .mci file belongs is read as the file name, as the
+    reference does (CUDAMCFLmain.cu:305-320): the parser fails and the exit
+    status is 1, before any GPU work."""
+    exe = os.path.join(M.BUILD_DIR, "mcfl")
+    p = subprocess.run([exe, "-S5"], capture_output=True, text=True)
+    assert p.returncode == 1 and "Something wrong with read_simulation_data!" in p.stdout
+    assert "No usable GPU" not in p.stdout
 
 
 def test_cli_rejects_unknown_argument(monkeypatch):
