@@ -834,6 +834,83 @@ void orc_backstep_test(const float* p, const float* d, const float* cc, float r2
   backstep(p[0], p[1], p[2], d[0], d[1], d[2], cc[0], cc[1], cc[2], r2, t1, t2);
 }
 
+/* n photons launched by orc_launch from the source (kind, s->xi, s->yi,
+ * s->zi) on photon streams: photon k uses multiplier a_tab[k & (P-1)] and
+ * the state mcfl_photon_state(key, k, a).  pos_dir[6k..6k+5] = x, y, z, dx,
+ * dy, dz; weight[k], label[k] the launch weight and entry label. */
+void orc_launch_sample(const orc_sim* s, int kind, const uint32_t* a_tab, uint32_t P, uint64_t key, uint64_t n,
+                       float* pos_dir, uint32_t* weight, uint32_t* label) {
+  ctx_t* c = malloc(sizeof(ctx_t));
+  ctx_init(c, s);
+  for (uint64_t k = 0; k < n; ++k) {
+    uint32_t a = a_tab[k & (P - 1u)];
+    uint64_t x = mcfl_photon_state(key, k, a);
+    orc_photon p;
+    orc_launch(c, kind, s->xi, s->yi, s->zi, &p, &x, a);
+    float* o = pos_dir + 6 * k;
+    o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.dx; o[4] = p.dy; o[5] = p.dz;
+    weight[k] = p.weight;
+    label[k] = p.bulkpos;
+  }
+  free(c);
+}
+
+/* survive() on a photon of weight w with generator state *x, multiplier a
+ * (advanced by the draws taken).  Returns 1 if the photon lives; *w_out its
+ * weight afterwards, gain/loss the roulette totals it adds. */
+int orc_survive_test(uint32_t w, uint64_t* x, uint32_t a, uint32_t* w_out, uint64_t* gain, uint64_t* loss) {
+  orc_photon p;
+  memset(&p, 0, sizeof p);
+  p.weight = w;
+  tl_t t;
+  memset(&t, 0, sizeof t);
+  int r = survive(&p, x, a, &t);
+  *w_out = p.weight;
+  *gain = t.tot[ORC_T_ROUL_GAIN];
+  *loss = t.tot[ORC_T_ROUL_LOSS];
+  return r;
+}
+
+/* survive() on n photons of weight w, photon k drawing from the stream
+ * (key, k) with multiplier a_tab[k & (P-1)]: the number that live and the
+ * sum of their weights afterwards. */
+void orc_survive_batch(uint32_t w, const uint32_t* a_tab, uint32_t P, uint64_t key, uint64_t n, uint64_t* lived,
+                       uint64_t* weight_sum) {
+  uint64_t nl = 0, sw = 0;
+  for (uint64_t k = 0; k < n; ++k) {
+    uint32_t a = a_tab[k & (P - 1u)];
+    uint64_t x = mcfl_photon_state(key, k, a);
+    uint32_t wo;
+    uint64_t g, l;
+    if (orc_survive_test(w, &x, a, &wo, &g, &l)) {
+      ++nl;
+      sw += wo;
+    }
+  }
+  *lived = nl;
+  *weight_sum = sw;
+}
+
+/* orc_move_to_boundary on a photon at pd[0..2] moving along pd[3..5] inside a
+ * region of label old_label.  Returns the label found; out[0..2] the
+ * position afterwards, *tof the time of flight it accumulated (from 2^40, so
+ * the back step cannot wrap). */
+uint32_t orc_march_test(const orc_sim* s, const float* pd, uint32_t old_label, float max_s, float* out,
+                        uint64_t* tof) {
+  ctx_t* c = malloc(sizeof(ctx_t));
+  ctx_init(c, s);
+  orc_photon p;
+  memset(&p, 0, sizeof p);
+  p.x = pd[0]; p.y = pd[1]; p.z = pd[2]; p.dx = pd[3]; p.dy = pd[4]; p.dz = pd[5];
+  p.bulkpos = old_label;
+  p.tof = 1ull << 40;
+  uint32_t lab = orc_move_to_boundary(c, &p, old_label, max_s);
+  out[0] = p.x; out[1] = p.y; out[2] = p.z;
+  *tof = p.tof - (1ull << 40);
+  free(c);
+  return lab;
+}
+
 /* ----------------------------------------------- math / generator exposure */
 
 void orc_math_eval(int fn, const float* in, float* out, size_t n) {

@@ -160,6 +160,18 @@ void orc_spin_test(float g, const float* dir_in, uint64_t* x, uint32_t a, float*
 void orc_backstep_test(const float* p, const float* d, const float* c, float r2, float* t1, float* t2);
 float orc_fresnel_test(float n1, float n2, float cos_i);
 
+/* orc_launch on photon streams (key, a_tab of P entries): n photons' position
+   and direction (6 floats each), weight and entry label. */
+void orc_launch_sample(const orc_sim* s, int kind, const uint32_t* a_tab, uint32_t P, uint64_t key, uint64_t n,
+                       float* pos_dir, uint32_t* weight, uint32_t* label);
+/* survive (PhotonSurvive): 1 if the photon of weight w lives; its weight after. */
+int orc_survive_test(uint32_t w, uint64_t* x, uint32_t a, uint32_t* w_out, uint64_t* gain, uint64_t* loss);
+/* survive on n photon streams of weight w: photons that live, sum of their weights. */
+void orc_survive_batch(uint32_t w, const uint32_t* a_tab, uint32_t P, uint64_t key, uint64_t n, uint64_t* lived,
+                       uint64_t* weight_sum);
+/* orc_move_to_boundary from position/direction pd: label found, position after, time of flight added. */
+uint32_t orc_march_test(const orc_sim* s, const float* pd, uint32_t old_label, float max_s, float* out,
+                        uint64_t* tof);
 /* fn: 0 log, 1 sin(2 pi t), 2 cos(2 pi t), 3 uint32 (bits of in) -> float toward zero, 4 1/sqrt,
    5 x*rsqrt(x), 6 renormalisation factor */
 void orc_math_eval(int fn, const float* in, float* out, size_t n);

@@ -274,7 +274,63 @@ def _bind_extras(L):
     L.orc_tof_div_mismatches.argtypes = [C.c_uint32, C.c_uint32]
     L.orc_tof_div_mismatches.restype = C.c_uint64
     L.orc_set_max_steps.argtypes = [C.c_uint32]
+    L.orc_launch_sample.argtypes = [C.POINTER(Sim), C.c_int, U32P, C.c_uint32, C.c_uint64, C.c_uint64, F32P, U32P,
+                                    U32P]
+    L.orc_launch_sample.restype = None
+    L.orc_survive_test.argtypes = [C.c_uint32, U64P, C.c_uint32, U32P, U64P, U64P]
+    L.orc_march_test.argtypes = [C.POINTER(Sim), F32P, C.c_uint32, C.c_float, F32P, U64P]
+    L.orc_march_test.restype = C.c_uint32
+    L.orc_survive_batch.argtypes = [C.c_uint32, U32P, C.c_uint32, C.c_uint64, C.c_uint64, U64P, U64P]
+    L.orc_survive_batch.restype = None
     L._extras = True
+
+
+def survive_batch(w, a_tab, key, n):
+    """PhotonSurvive on n photon streams of weight w: (photons that live, sum of their weights)."""
+    L = lib()
+    _bind_extras(L)
+    a_tab = np.ascontiguousarray(a_tab, np.uint32)
+    lived = np.zeros(1, np.uint64)
+    sw = np.zeros(1, np.uint64)
+    L.orc_survive_batch(w, ptr(a_tab, U32P), len(a_tab), key, n, ptr(lived, U64P), ptr(sw, U64P))
+    return int(lived[0]), int(sw[0])
+
+
+def launch_sample(sim, kind, a_tab, key, n):
+    """n photons of the oracle's LaunchPhoton on photon streams: (pos_dir
+    (n, 6) float32, weight, entry label)."""
+    L = lib()
+    _bind_extras(L)
+    a_tab = np.ascontiguousarray(a_tab, np.uint32)
+    pd = np.zeros((n, 6), np.float32)
+    w = np.zeros(n, np.uint32)
+    lab = np.zeros(n, np.uint32)
+    L.orc_launch_sample(C.byref(sim), kind, ptr(a_tab, U32P), len(a_tab), key, n, ptr(pd, F32P), ptr(w, U32P),
+                        ptr(lab, U32P))
+    return pd, w, lab
+
+
+def survive(w, x, a):
+    """The oracle's PhotonSurvive: (lives, weight after, state after, gain, loss)."""
+    L = lib()
+    _bind_extras(L)
+    xs = np.array([x], np.uint64)
+    wo = np.zeros(1, np.uint32)
+    g = np.zeros(1, np.uint64)
+    lo = np.zeros(1, np.uint64)
+    r = L.orc_survive_test(w, ptr(xs, U64P), a, ptr(wo, U32P), ptr(g, U64P), ptr(lo, U64P))
+    return r, int(wo[0]), int(xs[0]), int(g[0]), int(lo[0])
+
+
+def march(sim, pos, direction, old_label, max_s=100.0):
+    """The oracle's MoveToFirstBoundary: (label found, position after, time of flight added)."""
+    L = lib()
+    _bind_extras(L)
+    pd = np.array(list(pos) + list(direction), np.float32)
+    out = np.zeros(3, np.float32)
+    tof = np.zeros(1, np.uint64)
+    lab = L.orc_march_test(C.byref(sim), ptr(pd, F32P), old_label, max_s, ptr(out, F32P), ptr(tof, U64P))
+    return int(lab), out, int(tof[0])
 
 
 def math_eval(fn, x):

@@ -4,8 +4,10 @@ Each check comes from outside the code it tests: libm in double precision,
 the MWC closed form (n-th state = a^n x0 mod a*2^32-1, Python big integers),
 textbook Fresnel, Henyey-Greenstein moments, the reference notebook's
 back-step known answers (FoundBoundaryPython.ipynb), van de Hulst's published
-slab values, and Chandrasekhar's H-function for a semi-infinite isotropic
-medium.
+slab values, Chandrasekhar's H-function for a semi-infinite isotropic medium,
+the sources' closed-form distributions (Kolmogorov-Smirnov), the roulette's
+exact survival boundary and expectation, the glass march's geometry and an
+emission loop whose images are computed by hand.
 """
 import math
 import pathlib
@@ -558,3 +560,250 @@ def test_fixed_point_lane_emission_equals_reference_loop(a_tab):
         np.testing.assert_allclose(fx, img, rtol=1e-9, atol=1e-12 * img.max())
     with pytest.raises(ValueError):
         O.run_fluor_lanes(oe.o, x0.copy(), a, vox, scale, L - 1)
+
+
+# ------------------------------------- sources, roulette, glass march, emission loop
+
+def _ks_uniform(u):
+    """Kolmogorov-Smirnov distance of samples u from U[0, 1]."""
+    u = np.sort(np.asarray(u, np.float64))
+    n = len(u)
+    i = np.arange(1, n + 1)
+    return max(np.max(i / n - u), np.max(u - (i - 1) / n))
+
+
+KS_CRIT = 1.95  # D * sqrt(n) above this rejects at the 0.1% level
+
+
+def _source_medium(gs=10):
+    """8x8x6 voxels of 1/gs cm, labels 1 and 2 in a checkerboard of voxels."""
+    iz, iy, ix = np.indices((6, 8, 8))
+    lab = (1 + (ix + iy + iz) % 2).astype(np.uint8)
+    f32 = np.float32
+    b = [[FLT_MAX, 0, 1, 1.0, 0, 0, 0, 0], [f32(1) / f32(100.1), 0.1, 0.9, 1.4, 0, 0.1, 0.1, 1],
+         [f32(1) / f32(100.5), 0.5, 0.9, 1.4, 0, 0.5, 0.1, 1], [FLT_MAX, 0, 1, 1.0, 0, 0, 0, 0]]
+    return O.Medium(lab, b, 6 / gs, gs, halfx=8 / (2 * gs), halfy=8 / (2 * gs))
+
+
+def _expected_label(m, x, y, z):
+    """Label of the voxel holding each point (x fastest, CUDAMCFLtransport.cu:490-503)."""
+    ix = np.floor((x + float(m.halfx)) * m.gs).astype(np.int64)
+    iy = np.floor((y + float(m.halfy)) * m.gs).astype(np.int64)
+    iz = np.floor(z * m.gs).astype(np.int64)
+    return m.labels[iz, iy, ix]
+
+
+def test_pencil_source_radial_law(a_tab):
+    """The collimated source of LaunchPhoton (CUDAMCFLtransport.cu:507-543)
+    over 10^6 launches: the spot radius follows P(r < R) = 1 - exp(-R^2 /
+    0.03^2) and its azimuth is uniform (Kolmogorov-Smirnov against the
+    closed forms); every photon starts at z = zi moving along +z with the
+    start weight, labelled with the voxel it enters.  A radius
+    0.03 * (-ln xi), without the root, fails the radial test (checked by
+    planting it in a copy of the oracle)."""
+    m = _source_medium()
+    xi, yi, zi = np.float32(0.013), np.float32(-0.021), np.float32(0.0)
+    sim = m.sim(src=(1, float(xi), float(yi), float(zi), 0.0), start_weight=4123456789)
+    n = 1_000_000
+    pd, w, lab = O.launch_sample(sim, 1, a_tab, M.stream_key(41, 0), n)
+    x, y = pd[:, 0].astype(np.float64) - float(xi), pd[:, 1].astype(np.float64) - float(yi)
+    r2 = x * x + y * y
+    assert _ks_uniform(1.0 - np.exp(-r2 / 0.03 ** 2)) * math.sqrt(n) < KS_CRIT
+    phi = np.mod(np.arctan2(y, x), 2 * np.pi) / (2 * np.pi)
+    assert _ks_uniform(phi) * math.sqrt(n) < KS_CRIT
+    assert np.all(pd[:, 2] == zi) and np.all(pd[:, 3] == 0) and np.all(pd[:, 4] == 0) and np.all(pd[:, 5] == 1)
+    assert np.all(w == 4123456789)
+    assert np.array_equal(lab, _expected_label(m, pd[:, 0].astype(np.float64), pd[:, 1].astype(np.float64),
+                                               pd[:, 2].astype(np.float64)))
+    assert len(np.unique(lab)) == 2  # the spot covers voxels of both labels
+
+
+def test_isotropic_source_distributions(a_tab):
+    """The isotropic source of LaunchPhoton (CUDAMCFLtransport.cu:466-503) at
+    a voxel corner, over 10^6 launches: the position is uniform in
+    (xi, xi + 0.999999/gs] on each axis and never leaves the voxel, cos(theta)
+    is uniform on [-1, 1] and the azimuth on [0, 2 pi) (Kolmogorov-Smirnov),
+    the direction is a unit vector, the weight 0xFFFFFFFF and the label that
+    of the corner's voxel.  cos(theta) = 1 - xi instead of 1 - 2 xi fails the
+    cos(theta) test, and a position step 1/gs instead of 0.999999/gs fails
+    the bounds (both checked by planting them in a copy of the oracle)."""
+    m = _source_medium()
+    gs = m.gs
+    corner = np.array([-0.3, 0.2, 0.1], np.float32)  # voxel (1, 6, 1)
+    sim = m.sim(src=(0, float(corner[0]), float(corner[1]), float(corner[2]), 0.0))
+    n = 1_000_000
+    pd, w, lab = O.launch_sample(sim, 0, a_tab, M.stream_key(42, 0), n)
+    step = float(np.float32(np.float32(0.999999) / np.float32(gs)))
+    for k in range(3):
+        c = float(corner[k])
+        p = pd[:, k].astype(np.float64)
+        assert p.min() >= c and p.max() <= float(np.float32(c + step)) and p.max() < c + 1.0 / gs
+        assert _ks_uniform((p - c) / step) * math.sqrt(n) < KS_CRIT
+    d = pd[:, 3:6].astype(np.float64)
+    assert np.max(np.abs(np.sum(d * d, axis=1) - 1.0)) < 1e-6
+    assert _ks_uniform((d[:, 2] + 1.0) / 2.0) * math.sqrt(n) < KS_CRIT
+    psi = np.mod(np.arctan2(d[:, 1], d[:, 0]), 2 * np.pi) / (2 * np.pi)
+    assert _ks_uniform(psi) * math.sqrt(n) < KS_CRIT
+    assert np.all(w == 0xFFFFFFFF)
+    assert np.all(lab == m.labels[1, 6, 1])
+
+
+def _state_drawing(u, a):
+    """A generator state whose next [0,1) draw comes from the 32-bit value u:
+    carry 0 and low word u * a^-1 mod 2^32."""
+    return (u * pow(a, -1, 2 ** 32)) % 2 ** 32
+
+
+def test_roulette_rule_and_expectation(a_tab):
+    """PhotonSurvive (CUDAMCFLtransport.cu:850-864): weights above 429497
+    live without a draw, weight 0 dies without one; weights 1..429497 draw
+    once and live with probability 0.1f, their weight becoming w / 0.1
+    rounded, else die.  The survival boundary in the draw's 32-bit value is
+    found by bisection and equals 0.1f * 2^32 = 429496736 (0.1f is
+    13421773 * 2^-27), so E[w'] = w to within the rounding of w / 0.1 for
+    every weight tried; over 10^6 photon streams the mean surviving weight is
+    w within 4 sigma.  A rule `draw <= 0.1` moves the boundary by one float
+    step and fails; a weight w / 0.2 fails the expectation (both planted in
+    a copy of the oracle)."""
+    a = int(a_tab[0])
+    # no draw: above the threshold, or weight 0
+    for w, lives in ((429498, 1), (0xFFFFFFFF, 1), (0, 0)):
+        x0 = (123 << 32) | 456
+        r, wo, x1, g, lo = O.survive(w, x0, a)
+        assert (r, x1, g, lo) == (lives, x0, 0, 0) and wo == w
+    # the boundary of the draw (a u32 whose conversion toward zero times 2^-32 is the draw)
+    lo_u, hi_u = 0, 2 ** 32 - 1
+    assert O.survive(1000, _state_drawing(lo_u, a), a)[0] == 1 and O.survive(1000, _state_drawing(hi_u, a), a)[0] == 0
+    while hi_u - lo_u > 1:
+        mid = (lo_u + hi_u) // 2
+        if O.survive(1000, _state_drawing(mid, a), a)[0]:
+            lo_u = mid
+        else:
+            hi_u = mid
+    assert hi_u == 13421773 * 32 == 429496736
+    p_live = hi_u / 2 ** 32
+    for w in [1, 2, 3, 7, 1000, 12345, 65535, 99999, 300000, 429496, 429497]:
+        x0 = _state_drawing(hi_u - 1, a)
+        r, wo, x1, g, lo = O.survive(w, x0, a)
+        assert r == 1 and abs(wo - 10 * w) <= 1 and g == wo - w and lo == 0 and x1 != x0
+        assert abs(p_live * wo - w) <= 0.1 + 2e-8 * w
+        r, wo, x1, g, lo = O.survive(w, _state_drawing(hi_u, a), a)
+        assert r == 0 and wo == 0 and lo == w and g == 0
+    n, w = 1_000_000, 300000
+    lived, total = O.survive_batch(w, a_tab, M.stream_key(43, 0), n)
+    sigma = 10 * w * math.sqrt(0.1 * 0.9 / n)
+    assert abs(total / n - w) < 4 * sigma
+    assert abs(lived / n - 0.1) < 4 * math.sqrt(0.09 / n)
+
+
+def _glass_medium():
+    """8x8x10 voxels of 1 mm: scattering label 1 with a glass layer (label 2,
+    mus = 0) at voxel depths 2..6."""
+    lab = np.ones((10, 8, 8), np.uint8)
+    lab[2:7] = 2
+    f32 = np.float32
+    b = [[FLT_MAX, 0, 1, 1.0, 0, 0, 0, 0], [f32(1) / f32(100.1), 0.1, 0.9, 1.4, 0, 0.1, 0.1, 1],
+         [FLT_MAX, 0.0, 1.0, 1.5, 0, 0, 0, 1], [FLT_MAX, 0, 1, 1.0, 0, 0, 0, 0]]
+    return O.Medium(lab, b, 1.0, 10, halfx=0.4, halfy=0.4)
+
+
+def _march_geometry(m, p0, d, h):
+    """The half-voxel march of MoveToFirstBoundary in double precision
+    (CUDAMCFLtransport.cu:717-756): position and voxel (ix, iy, iz) of the
+    first step that lands in a voxel of another label, and the step count."""
+    p = np.array(p0, np.float64)
+    for k in range(1, 1000):
+        p = p + h * d
+        nxt = p + h * d
+        assert 0.0 <= nxt[2] <= float(m.esp) and abs(nxt[0]) < float(m.halfx) and abs(nxt[1]) < float(m.halfy)
+        v = np.floor((p + [float(m.halfx), float(m.halfy), 0.0]) * m.gs).astype(int)
+        frac = (p + [float(m.halfx), float(m.halfy), 0.0]) * m.gs - v
+        assert np.all(np.minimum(frac, 1 - frac) > 1e-4)  # no march point near a voxel face
+        if m.labels[v[2], v[1], v[0]] != 2:
+            return p, v, k
+    raise AssertionError("no boundary")
+
+
+def test_glass_march_lands_on_the_backoff_sphere():
+    """MoveToFirstBoundary (CUDAMCFLtransport.cu:702-847) through a glass
+    layer: the photon marches in half-voxel steps (the time of flight grows by
+    trunc(0.05 cm / c) per step) to the first voxel of another label, whose
+    label it returns, then backs off along its ray onto the sphere of radius
+    0.707107/gs around that voxel's centre.  The expected voxel, step count
+    and the ray's two crossings of that sphere are computed here from the
+    geometry in double precision; the landing point is one of the crossings.
+    A march of whole voxels fails the time of flight; a sphere of radius
+    0.5/gs (the reference comment's value) misses both crossings (both
+    planted in a copy of the oracle)."""
+    m = _glass_medium()
+    sim = m.sim()
+    c_cmfs = 2.99792e-5
+    h = float(np.float32(1.0) / (np.float32(2.0) * np.float32(10.0)))
+    cases = [((0.013, -0.027, 0.234), (0.3, -0.2, 0.93), 1),   # down into the scattering layer below
+             ((0.052, 0.031, 0.612), (-0.25, 0.4, -0.88), 1),  # up into the layer above
+             ((-0.151, 0.143, 0.466), (0.55, 0.45, 0.704), 1)]  # steeply, through several columns
+    for p0, d0, want in cases:
+        d = np.array(d0, np.float32)
+        d = (d / np.float32(np.sqrt(np.sum(d.astype(np.float64) ** 2)))).astype(np.float32)
+        p0 = np.array(p0, np.float32)
+        lab, out, tof = O.march(sim, p0, d, 2)
+        pk, v, k = _march_geometry(m, p0.astype(np.float64), d.astype(np.float64), h)
+        assert lab == want == m.labels[v[2], v[1], v[0]]
+        c = (v + 0.5) / m.gs - [float(m.halfx), float(m.halfy), 0.0]
+        r = 0.707107 / m.gs
+        dd = d.astype(np.float64)
+        q = pk - c
+        b = dd @ q
+        disc = b * b - (dd @ dd) * (q @ q - r * r)
+        assert disc > 0
+        s = [(-b - math.sqrt(disc)) / (dd @ dd), (-b + math.sqrt(disc)) / (dd @ dd)]
+        cands = [pk + si * dd for si in s]
+        dist = [np.max(np.abs(out - cc)) for cc in cands]
+        j = int(np.argmin(dist))
+        assert dist[j] < 2e-6, (dist, out, cands)
+        back = -s[j]  # the reference moves by p - t d
+        step_t = int(h / c_cmfs)
+        assert abs(tof - (k * step_t - max(0, int(back / c_cmfs)))) <= 1, (tof, k)
+
+
+def test_emission_loop_images_by_hand(a_tab):
+    """The reference's emission loop (CUDAMCFLmain.cu:471-594) on a 3x1x1
+    medium whose photons all leave on their first step (mean free path
+    10^7 cm in a 1 mm slab, matched indices, one 10^6 cm detector pixel):
+    each emitting voxel adds eY (1 - albedof) Fx R / (N (2^32 - 1) dx dy) to
+    FlR (likewise T to FlT, :552-577), and with every photon leaving through
+    the top or the bottom at full weight the two images sum to
+    sum_v eY_v (1 - albedof_v) Fx_v / (dx dy), computed here by hand; the
+    voxel of a material with albedof = 1 emits nothing but counts one
+    photon (:558-561); each emitting voxel's photons split evenly between
+    top and bottom (isotropic source) within 4 sigma.  Dropping (1 - albedof)
+    or dividing by N twice fails the sum (both planted in a copy of the
+    oracle)."""
+    lab = np.array([[[1, 2, 3]]], np.uint8)
+    f32 = np.float32
+    mutr = float(f32(1) / f32(1e-7))
+    b = [[FLT_MAX, 0, 1, 1.0, 0, 0, 0, 0],
+         [mutr, 0.0, 0.9, 1.0, 1.0, 0.0, 0.1, 0.5],
+         [mutr, 0.0, 0.9, 1.0, 1.0, 0.0, 0.3, 0.25],
+         [mutr, 0.0, 0.9, 1.0, 1.0, 0.0, 0.7, 1.0],
+         [FLT_MAX, 0, 1, 1.0, 0, 0, 0, 0]]
+    m = O.Medium(lab, b, 0.1, 10, halfx=0.15, halfy=0.05)
+    sim = m.sim(det=(1e6, 1e6, 0.0, 0.0, 1, 1), src=(0, 0.0, 0.0, 0.0, 0.0))
+    Fx = np.array([0.37, 0.81, 0.55])
+    nv, L = 4096, 64
+    x, a = M.init_rng(L, seed=8)
+    FlR, FlT, n = O.fluor_reference(sim, Fx, x, a, nv)
+    assert n == 2 * nv + 1
+    dxdy = float(f32(1e6)) ** 2
+    eY1, eY2 = float(f32(0.1)), float(f32(0.3))
+    hand = (eY1 * (1 - 0.5) * 0.37 + eY2 * (1 - 0.25) * 0.81) / dxdy
+    assert abs((FlR.sum() + FlT.sum()) / hand - 1) < 1e-12
+    # top/bottom split of each voxel's photons (one voxel at a time)
+    for v, (eY, af) in ((0, (eY1, 0.5)), (1, (eY2, 0.25))):
+        F1 = np.zeros(3)
+        F1[v] = 1.0
+        x, a = M.init_rng(L, seed=9 + v)
+        R, T, _ = O.fluor_reference(sim, F1, x, a, nv)
+        tot = eY * (1 - af) / dxdy
+        assert abs((R.sum() + T.sum()) / tot - 1) < 1e-12
+        assert abs(R.sum() / tot - 0.5) < 4 * 0.5 / math.sqrt(nv)
