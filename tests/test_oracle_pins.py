@@ -283,6 +283,62 @@ def test_spin_henyey_greenstein_moments(g):
     assert abs((mu ** 2).mean() - m2) < 4 * (mu ** 2).std() / math.sqrt(n) + 1e-4
 
 
+@pytest.mark.parametrize("g", [0.0, 0.5, 0.9])
+def test_spin_equals_the_reference_rotation(g):
+    """Spin (CUDAMCFLtransport.cu:585-625) written out here in double
+    precision from the reference's own expressions: cos(theta) from the
+    Henyey-Greenstein inversion (or 2 xi - 1 from an extra draw when g = 0),
+    psi = 2 pi xi with the sine assigned to cosp and the cosine to sinp
+    (:603), the normal-incidence branch when 1 - dz^2 rounds to 0 (:607-612)
+    and the general rotation otherwise (:613-618), then renormalised.  The
+    draws are the generator's (pinned by its closed form).  The oracle's
+    FP32 rewrite (one reciprocal root for sint/temp and sint*temp, a
+    first-order renormalisation) agrees to 3e-6 + 1e-6/sin(theta) (FP32's
+    own conditioning near cos(theta) = +-1) on every component for 4000
+    directions, among them dz = +-1 and dz within 2^-12 of +-1.  Swapping
+    the sine and cosine back, or the sign of the dz term, fails it."""
+    import ctypes as C
+    lib = O.lib()
+    a = 4294966893
+    rng = np.random.default_rng(17)
+    dirs = rng.normal(size=(4000, 3))
+    dirs[:8] = [[0, 0, 1], [0, 0, -1], [1e-4, 0, 1], [0, -3e-4, -1], [0.6, 0, 0.8], [0, 1, 0], [1, 0, 0],
+                [0.3, 0.4, 0]]
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    g32 = float(np.float32(g))
+    worst = 0.0
+    for i, d in enumerate(dirs.astype(np.float32)):
+        x0 = ((i + 3) << 32) | (2654435761 * (i + 1) % 2 ** 32)
+        _, draws = O.mwc_draws(x0, a, 3)
+        dr = draws.astype(np.float64)
+        if g32 != 0.0:
+            t = (1 - g32 * g32) / (1 - g32 + 2 * g32 * dr[0])
+            cost = (1 + g32 * g32 - t * t) / (2 * g32)
+            u = dr[1]
+        else:
+            cost = 2 * dr[1] - 1
+            u = dr[2]
+        psi = 2 * math.pi * u
+        cosp, sinp = math.sin(psi), math.cos(psi)  # the reference's argument order
+        sint = math.sqrt(max(0.0, 1 - cost * cost))
+        dx, dy, dz = map(float, d)
+        if np.float32(1) - np.float32(dz) * np.float32(dz) <= 0:
+            nd = np.array([sint * cosp, sint * sinp, math.copysign(cost, dz * cost)])
+        else:
+            temp = math.sqrt(1 - dz * dz)
+            nd = np.array([sint * (dx * dz * cosp - dy * sinp) / temp + dx * cost,
+                           sint * (dy * dz * cosp + dx * sinp) / temp + dy * cost,
+                           -sint * cosp * temp + dz * cost])
+        nd /= np.linalg.norm(nd)
+        xs = C.c_uint64(x0)
+        dout = np.zeros(3, np.float32)
+        lib.orc_spin_test(g, O.ptr(np.ascontiguousarray(d), O.F32P), C.byref(xs), a, O.ptr(dout, O.F32P))
+        # FP32 conditioning: near cos(theta) = +-1 an error e in cos(theta) becomes e / sin(theta) in sin(theta)
+        tol = 3e-6 + 1e-6 / max(sint, 1e-6)
+        worst = max(worst, float(np.max(np.abs(dout.astype(np.float64) - nd))) / tol)
+    assert worst < 1.0, worst
+
+
 def test_backstep_known_answers():
     import ctypes as C
     lib = O.lib()
