@@ -96,7 +96,7 @@ def rse_stats(jobs):
     vals = []
     for j in jobs:
         S = j.Rd.astype(np.float64)
-        Q = j.Rd2.astype(np.float64) * 2.0 ** 24
+        Q = j.Rd2.astype(np.float64) * 2.0 ** 32
         keep = S > 1e-4 * S.max()
         r = np.sqrt(np.maximum(Q[keep] - S[keep] ** 2 / j.photons, 0.0)) / S[keep]
         vals.append(r)
@@ -331,7 +331,7 @@ def bench_sample2(ctx, frac=1.0):
     xy = s.det.nx * s.det.ny
     k = job.fl_unit / (float(np.float32(s.det.dx)) * float(np.float32(s.det.dy)))
     is_tot = float(job.Rd[:xy].astype(np.float64).sum()) * k
-    is_sig = float(np.sqrt(job.Rd2[:xy].astype(np.float64).sum() * 2.0 ** 24)) * k
+    is_sig = float(np.sqrt(job.Rd2[:xy].astype(np.float64).sum() * 2.0 ** 32)) * k
     return {"workload": f"sample2.mci ({s.nx}x{s.ny}x{s.nz} voxels from 3dbulk-sample, temporal detectors), "
                         f"photon counts x{frac:g}",
             "excitation": {"photons": n_ex, "device_ms": ex.kernel_ms, "wall_s": t1 - t0,

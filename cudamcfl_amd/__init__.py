@@ -91,7 +91,8 @@ _lib = None
 
 EXPORTS = ["mcfl_read_simulation_data", "mcfl_free_simulations", "mcfl_interpret_arg", "mcfl_init_rng",
            "mcfl_default_primes_path", "mcfl_stream_key", "mcfl_ctx_create", "mcfl_ctx_destroy",
-           "mcfl_status_string", "mcfl_ctx_info", "mcfl_run_excitation", "mcfl_DoOneSimulation", "mcfl_fl_plan",
+           "mcfl_status_string", "mcfl_ctx_info", "mcfl_run_excitation", "mcfl_DoOneSimulation",
+           "mcfl_DoOneSimulationFl", "mcfl_fl_plan",
            "mcfl_run_fluorescence", "mcfl_run_fluorescence_lanes", "mcfl_write_simulation_results", "mcfl_write_phd",
            "mcfl_write_fl"]
 
@@ -121,6 +122,8 @@ def lib():
     L.mcfl_run_excitation.argtypes = [C.c_void_p, C.POINTER(Sim), C.POINTER(Rng), C.POINTER(Tallies)]
     L.mcfl_DoOneSimulation.argtypes = [C.c_void_p, C.POINTER(Sim), U64P, U32P, F64P]
     L.mcfl_DoOneSimulation.restype = C.c_uint64
+    L.mcfl_DoOneSimulationFl.argtypes = [C.c_void_p, C.POINTER(Sim), U64P, U32P, U64P, U64P]
+    L.mcfl_DoOneSimulationFl.restype = C.c_uint64
     L.mcfl_fl_plan.argtypes = [C.POINTER(Sim), F64P, C.c_uint64, U32P, F64P, U64P, F64P]
     L.mcfl_run_fluorescence.argtypes = [C.c_void_p, C.POINTER(Sim), F64P, C.c_uint64, C.c_uint64, F64P, F64P,
                                         C.POINTER(FlResult)]
@@ -412,6 +415,16 @@ class Context:
         rng.first_photon = int(first_photon)  # the library refuses an offset on lane streams
         check(lib().mcfl_run_excitation(self.h, C.byref(sim.s), C.byref(rng), C.byref(r.t)), "run_excitation")
         return r
+
+    def DoOneSimulationFl(self, sim, x, a):
+        """mcfl_DoOneSimulationFl: one emission simulation of the reference's
+        voxel loop on lanes (x, a) (x advanced in place), with sim as the
+        caller set it up.  Returns (photons finished, R, T); 0 photons on
+        failure."""
+        R = np.zeros(sim.npix, np.uint64)
+        T = np.zeros(sim.npix, np.uint64)
+        n = lib().mcfl_DoOneSimulationFl(self.h, C.byref(sim.s), _p(x, U64P), _p(a, U32P), _p(R, U64P), _p(T, U64P))
+        return int(n), R, T
 
     def run_fluorescence_lanes(self, sim, Fx, n_per_voxel, lanes):
         """mcfl_run_fluorescence_lanes: the voxel loop on the reference's lane streams

@@ -175,6 +175,9 @@ struct HostPool {
 
 struct Tallies {
   DevBuf Rd, Tt, fhd, axyz, Rd2, Tt2, time, tot, al_prob, al_alias, al_vox;
+  // Volume tallies of a job run in slices: each slice's per-XCD copies are
+  // summed into these (saturating at 2^64-1) and cleared for the next slice.
+  DevBuf fhd_acc, axyz_acc;
   uint32_t flags = 0;
   uint32_t copies = 1; /* per-XCD copies of fhd / axyz */
   uint32_t al_n = 0;
@@ -183,7 +186,8 @@ struct Tallies {
   BufPool* pool = nullptr; /* served requests: buffers come from and go back to the server's pool */
   mcfl_status get(DevBuf& b, size_t n) { return pool ? pool->get(b, n) : ensure(b, n); }
   ~Tallies() {
-    for (DevBuf* b : {&Rd, &Tt, &fhd, &axyz, &Rd2, &Tt2, &time, &tot, &al_prob, &al_alias, &al_vox}) {
+    for (DevBuf* b : {&Rd, &Tt, &fhd, &axyz, &Rd2, &Tt2, &time, &tot, &al_prob, &al_alias, &al_vox, &fhd_acc,
+                      &axyz_acc}) {
       if (pool)
         pool->put(*b);
       else
@@ -239,6 +243,33 @@ int reduce_tallies(const Medium& m, Tallies* t, hipStream_t st) {
   int e = 0;
   if (t->fhd.p) e |= mcfl_reduce_copies(t->fhd.as<unsigned long long>(), m.nvox, t->copies, st);
   if (t->axyz.p) e |= mcfl_reduce_copies(t->axyz.as<unsigned long long>(), m.nvox, t->copies, st);
+  return e;
+}
+
+// Volume tallies run in slices: allocate the accumulators (cleared on st).
+mcfl_status alloc_accumulators(const Medium& m, Tallies* t, hipStream_t st) {
+  if (t->fhd.p) {
+    MCFL_STCHK(t->get(t->fhd_acc, m.nvox * 8));
+    if (mcfl_fill_u64(t->fhd_acc.as<unsigned long long>(), m.nvox, st) != 0) return MCFL_ERR_DEVICE;
+    t->bytes += m.nvox * 8;
+  }
+  if (t->axyz.p) {
+    MCFL_STCHK(t->get(t->axyz_acc, m.nvox * 8));
+    if (mcfl_fill_u64(t->axyz_acc.as<unsigned long long>(), m.nvox, st) != 0) return MCFL_ERR_DEVICE;
+    t->bytes += m.nvox * 8;
+  }
+  return MCFL_OK;
+}
+
+// A slice's per-XCD copies into the accumulators (saturating), copies cleared.
+int accumulate_tallies(const Medium& m, Tallies* t, hipStream_t st) {
+  int e = 0;
+  if (t->fhd_acc.p)
+    e |= mcfl_accumulate_copies(t->fhd.as<unsigned long long>(), m.nvox, t->copies,
+                                t->fhd_acc.as<unsigned long long>(), st);
+  if (t->axyz_acc.p)
+    e |= mcfl_accumulate_copies(t->axyz.as<unsigned long long>(), m.nvox, t->copies,
+                                t->axyz_acc.as<unsigned long long>(), st);
   return e;
 }
 
@@ -354,8 +385,9 @@ mcfl_status download(const Medium& m, const Tallies& t, uint64_t* Rd, uint64_t* 
   const size_t npix = m.npix * 8, nvox = m.nvox * 8;
   if (Rd) MCFL_HIPCHK(hipMemcpy(Rd, t.Rd.p, npix, hipMemcpyDeviceToHost));
   if (Tt) MCFL_HIPCHK(hipMemcpy(Tt, t.Tt.p, npix, hipMemcpyDeviceToHost));
-  if (fhd && t.fhd.p) MCFL_HIPCHK(hipMemcpy(fhd, t.fhd.p, nvox, hipMemcpyDeviceToHost));
-  if (axyz && t.axyz.p) MCFL_HIPCHK(hipMemcpy(axyz, t.axyz.p, nvox, hipMemcpyDeviceToHost));
+  if (fhd && t.fhd.p) MCFL_HIPCHK(hipMemcpy(fhd, (t.fhd_acc.p ? t.fhd_acc : t.fhd).p, nvox, hipMemcpyDeviceToHost));
+  if (axyz && t.axyz.p)
+    MCFL_HIPCHK(hipMemcpy(axyz, (t.axyz_acc.p ? t.axyz_acc : t.axyz).p, nvox, hipMemcpyDeviceToHost));
   if (Rd2 && t.Rd2.p) MCFL_HIPCHK(hipMemcpy(Rd2, t.Rd2.p, npix, hipMemcpyDeviceToHost));
   if (Tt2 && t.Tt2.p) MCFL_HIPCHK(hipMemcpy(Tt2, t.Tt2.p, npix, hipMemcpyDeviceToHost));
   if (tim && t.time.p) MCFL_HIPCHK(hipMemcpy(tim, t.time.p, ntime_of(&m.sim) * 8, hipMemcpyDeviceToHost));
@@ -420,6 +452,12 @@ struct mcfl_batch {
   int temporal = 0, perjob = 0, nblocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // A job above the context's photon split (mcfl_ctx::run_photons) makes the
+  // batch run in slices: slice r runs photons [r S, (r+1) S) of every job
+  // that has them, the volume tallies summed into saturating accumulators
+  // between slices and the totals on the host, as mcfl_run_excitation
+  // splits a long run.
+  bool sliced = false;
 };
 
 namespace {
@@ -469,6 +507,8 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
     const mcfl_job& s = jobs[j];
     if (s.medium < 0 || s.medium >= n_media || s.photons == 0) return MCFL_ERR_ARG;
     if (s.src_kind == MCFL_SOURCE_EMISSION && !s.Fx) return MCFL_ERR_ARG;
+    if (s.photons > 0xFFFFFFFFull) return MCFL_ERR_CAPACITY;  // images stay exact below 2^32 photons
+    if (s.photons > c->run_photons) b->sliced = true;
     const Medium& m = *b->media[s.medium];
     used[s.medium] = true;
     b->tal.emplace_back(new Tallies());
@@ -497,6 +537,9 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
   MCFL_STCHK(b->L.upload(b->jd, c->stream));
   b->nblocks = grid_blocks(c, b->A, b->temporal, b->perjob);
   b->A.n_waves = (uint32_t)b->nblocks * (mcfl_wg_of(&b->A) / 64u);
+  if (b->sliced)
+    for (size_t j = 0; j < b->jd.size(); ++j)
+      MCFL_STCHK(alloc_accumulators(*b->media[b->spec[j].medium], b->tal[j].get(), c->stream));
   MCFL_HIPCHK(hipStreamSynchronize(c->stream));
   MCFL_STCHK(capture(b.get()));
   *out = b.release();
@@ -640,6 +683,77 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
   return MCFL_OK;
 }
 
+namespace {
+// A batch in slices of at most S photons per job (see mcfl_batch::sliced):
+// the same photon streams as one launch, so the same tallies, except that
+// volume tallies and totals saturate at 2^64-1 instead of wrapping.
+mcfl_status run_sliced(mcfl_batch* b, mcfl_job* jobs, double* kernel_ms) {
+  mcfl_ctx* c = b->c;
+  const uint64_t S = c->run_photons;
+  const size_t n = b->jd.size();
+  uint64_t rounds = 0;
+  for (const JobDev& J : b->jd) rounds = std::max<uint64_t>(rounds, (J.count + S - 1) / S);
+  for (size_t j = 0; j < n; ++j) {
+    const Medium& m = *b->media[b->spec[j].medium];
+    Tallies* t = b->tal[j].get();
+    MCFL_STCHK(clear_tallies(m, t, c->stream));
+    if ((t->fhd_acc.p && mcfl_fill_u64(t->fhd_acc.as<unsigned long long>(), m.nvox, c->stream) != 0) ||
+        (t->axyz_acc.p && mcfl_fill_u64(t->axyz_acc.as<unsigned long long>(), m.nvox, c->stream) != 0))
+      return MCFL_ERR_DEVICE;
+  }
+  std::vector<std::vector<uint64_t>> tot(n, std::vector<uint64_t>(MCFL_NTOT, 0));
+  std::vector<uint64_t> part(MCFL_NTOT);
+  std::vector<JobDev> jr(b->jd);
+  double ms_all = 0.0;
+  for (uint64_t r = 0; r < rounds; ++r) {
+    uint64_t begin = 0;
+    for (size_t j = 0; j < n; ++j) {
+      const uint64_t full = b->jd[j].count, k0 = r * S;
+      jr[j] = b->jd[j];
+      jr[j].k0 = k0;
+      jr[j].count = full > k0 ? std::min(S, full - k0) : 0;
+      jr[j].begin = begin;
+      begin += jr[j].count;
+    }
+    LaunchArgs A = b->A;
+    A.total = begin;
+    MCFL_HIPCHK(hipMemcpyAsync(b->L.d_jobs.p, jr.data(), n * sizeof(JobDev), hipMemcpyHostToDevice, c->stream));
+    if (mcfl_fill_u64(A.counter, A.njobs, c->stream) != 0) return MCFL_ERR_DEVICE;
+    MCFL_HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (mcfl_transport_launch(&A, b->temporal, b->perjob, b->nblocks, c->stream) != 0) return MCFL_ERR_DEVICE;
+    for (size_t j = 0; j < n; ++j)
+      if (jr[j].count > 0 && accumulate_tallies(*b->media[b->spec[j].medium], b->tal[j].get(), c->stream) != 0)
+        return MCFL_ERR_DEVICE;
+    MCFL_HIPCHK(hipEventRecord(c->ev1, c->stream));
+    MCFL_HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    MCFL_HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    ms_all += ms;
+    for (size_t j = 0; j < n; ++j) {  // totals of the slice, added saturating
+      if (jr[j].count == 0) continue;
+      Tallies* t = b->tal[j].get();
+      MCFL_HIPCHK(hipMemcpy(part.data(), t->tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost));
+      for (int i = 0; i < MCFL_NTOT; ++i) {
+        const uint64_t a = tot[j][i] + part[i];
+        tot[j][i] = a < part[i] ? ~0ull : a;
+      }
+      if (mcfl_fill_u64(t->tot.as<unsigned long long>(), MCFL_NTOT, c->stream) != 0) return MCFL_ERR_DEVICE;
+    }
+  }
+  MCFL_HIPCHK(hipMemcpyAsync(b->L.d_jobs.p, b->jd.data(), n * sizeof(JobDev), hipMemcpyHostToDevice, c->stream));
+  MCFL_HIPCHK(hipStreamSynchronize(c->stream));
+  if (kernel_ms) *kernel_ms = ms_all;
+  for (size_t j = 0; j < n; ++j) {
+    mcfl_job& o = jobs[j];
+    const Medium& m = *b->media[b->spec[j].medium];
+    MCFL_STCHK(download(m, *b->tal[j], o.Rd_xy, o.Tt_xy, o.fhd, o.A_xyz, o.Rd2_xy, o.Tt2_xy, o.time_xyt, nullptr));
+    memcpy(o.totals, tot[j].data(), sizeof o.totals);
+    o.fl_unit = b->tal[j]->fl_unit;
+  }
+  return MCFL_OK;
+}
+}  // namespace
+
 extern "C" mcfl_status mcfl_batch_run(mcfl_batch* b, mcfl_job* jobs, double* kernel_ms) {
   if (!b || !jobs) return MCFL_ERR_ARG;
   mcfl_ctx* c = b->c;
@@ -652,6 +766,7 @@ extern "C" mcfl_status mcfl_batch_run(mcfl_batch* b, mcfl_job* jobs, double* ker
       changed = true;
     }
   }
+  if (b->sliced) return run_sliced(b, jobs, kernel_ms);
   if (changed)
     MCFL_HIPCHK(hipMemcpyAsync(b->L.d_jobs.p, b->jd.data(), b->jd.size() * sizeof(JobDev), hipMemcpyHostToDevice,
                                c->stream));
@@ -714,8 +829,17 @@ struct Request {
   hipEvent_t ev = nullptr;
   uint8_t* stage = nullptr;
   size_t stage_bytes = 0;
+  // Excitation of a fluorescence request above the photon split: its fhd
+  // copies take at most S photons (mcfl_ctx::run_photons) between two sums
+  // into the saturating accumulator ex->fhd_acc.  since_acc counts the
+  // photons handed out since the last sum; acc_ev marks the sum, which the
+  // next epoch carrying the request waits for (acc_wait).
+  uint64_t since_acc = 0;
+  hipEvent_t acc_ev = nullptr;
+  bool acc_wait = false;
   ~Request() {
     if (ev) (void)hipEventDestroy(ev);
+    if (acc_ev) (void)hipEventDestroy(acc_ev);
     if (stage) (void)hipHostFree(stage);  // normally handed back to the server's pool at retirement
   }
   size_t img_bytes() const { return m->npix * 8; }
@@ -723,9 +847,10 @@ struct Request {
   uint8_t* stage_vol() const { return stage_tot() + MCFL_NTOT * 8; }
 };
 
-size_t footprint_of(const Medium& m, const mcfl_request& r, uint32_t copies) {
+size_t footprint_of(const Medium& m, const mcfl_request& r, uint32_t copies, uint64_t split) {
   size_t b = 2 * m.npix * 8 + MCFL_NTOT * 8;
   if (r.fluorescence) b += m.nvox * 8 * copies + 2 * m.npix * 8 + MCFL_NTOT * 8 + 3 * 4 * m.nvox;
+  if (r.fluorescence && r.photons > split) b += m.nvox * 8;  // the fhd accumulator
   return b;
 }
 
@@ -826,7 +951,8 @@ extern "C" mcfl_status mcfl_server_submit(mcfl_server* s, const mcfl_request* r,
   q->id = s->next_id++;
   q->r = *r;
   q->m = s->media[r->medium].get();
-  q->footprint = footprint_of(*q->m, *r, s->c->copies);
+  if (r->photons > 0xFFFFFFFFull) return MCFL_ERR_CAPACITY;  // images and totals stay exact below 2^32 photons
+  q->footprint = footprint_of(*q->m, *r, s->c->copies, s->c->run_photons);
   if (q->footprint > s->budget) return MCFL_ERR_CAPACITY;
   *id = q->id;
   s->queue.push_back(std::move(q));
@@ -845,6 +971,7 @@ mcfl_status admit(mcfl_server* s) {
     const hipStream_t st = s->ep[s->next].stream;
     MCFL_STCHK(alloc_tallies(*q.m, q.r.fluorescence ? (uint32_t)MCFL_TALLY_FHD : 0u, false, s->c->copies, q.ex.get()));
     MCFL_STCHK(clear_tallies(*q.m, q.ex.get(), st));
+    if (q.r.fluorescence && q.r.photons > s->c->run_photons) MCFL_STCHK(alloc_accumulators(*q.m, q.ex.get(), st));
     if (q.r.fluorescence) {
       q.em.reset(new Tallies());
       q.em->pool = &s->pool;
@@ -873,8 +1000,13 @@ mcfl_status start_copy_out(HostPool& pool, Request& q, hipStream_t st) {
   }
   if (!q.ev) MCFL_HIPCHK(hipEventCreateWithFlags(&q.ev, hipEventDisableTiming));
   if (to_emission) {
-    if (reduce_tallies(m, &t, st) != 0) return MCFL_ERR_DEVICE;
-    MCFL_HIPCHK(hipMemcpyAsync(q.stage_vol(), t.fhd.p, m.nvox * 8, hipMemcpyDeviceToHost, st));
+    if (t.fhd_acc.p) {  // the last slice into the accumulator
+      if (accumulate_tallies(m, &t, st) != 0) return MCFL_ERR_DEVICE;
+      MCFL_HIPCHK(hipMemcpyAsync(q.stage_vol(), t.fhd_acc.p, m.nvox * 8, hipMemcpyDeviceToHost, st));
+    } else {
+      if (reduce_tallies(m, &t, st) != 0) return MCFL_ERR_DEVICE;
+      MCFL_HIPCHK(hipMemcpyAsync(q.stage_vol(), t.fhd.p, m.nvox * 8, hipMemcpyDeviceToHost, st));
+    }
   }
   MCFL_HIPCHK(hipMemcpyAsync(q.stage, t.Rd.p, q.img_bytes(), hipMemcpyDeviceToHost, st));
   MCFL_HIPCHK(hipMemcpyAsync(q.stage + q.img_bytes(), t.Tt.p, q.img_bytes(), hipMemcpyDeviceToHost, st));
@@ -988,10 +1120,21 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
     const Medium* uni = nullptr;
     bool perjob = false;
     uint64_t begin = 0;
+    std::vector<hipEvent_t> waits;
     for (std::unique_ptr<Request>& qp : s->active) {
       Request& q = *qp;
       if ((q.phase != 1 && q.phase != 2) || q.done >= q.r.photons) continue;
-      const uint64_t take = share[q.id];
+      uint64_t take = share[q.id];
+      const bool acc = q.phase == 1 && q.ex->fhd_acc.p;
+      if (acc) {  // at most S photons into the fhd copies between two sums
+        take = std::min(take, c->run_photons - std::min(c->run_photons, q.since_acc));
+        if (take == 0) continue;  // waits for the sum queued when its last epoch completes
+        q.since_acc += take;
+        if (q.acc_wait) {
+          waits.push_back(q.acc_ev);
+          q.acc_wait = false;
+        }
+      }
       if (!uni) uni = q.m;
       if (q.m != uni) perjob = true;
       const bool em = q.phase == 2;
@@ -1017,6 +1160,7 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
       const int nblocks = std::max(1, grid_blocks(c, A, 0, perjob ? 1 : 0) - s->reserve);
       A.n_waves = (uint32_t)nblocks * (mcfl_wg_of(&A) / 64u);
       MCFL_HIPCHK(hipMemsetAsync(A.counter, 0, 8 * jd.size(), E.stream));
+      for (hipEvent_t w : waits) MCFL_HIPCHK(hipStreamWaitEvent(E.stream, w, 0));
       MCFL_HIPCHK(hipEventRecord(E.e0, E.stream));
       if (mcfl_transport_launch(&A, 0, perjob ? 1 : 0, nblocks, E.stream) != 0) return MCFL_ERR_DEVICE;
       MCFL_HIPCHK(hipEventRecord(E.e1, E.stream));
@@ -1048,6 +1192,25 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
     for (const auto& pt : O.parts)
       for (std::unique_ptr<Request>& qp : s->active)
         if (qp->id == pt.first) qp->finished += pt.second;
+    // requests whose fhd copies hold S photons since their last sum, with no
+    // epoch in flight still adding to them and photons left to hand out:
+    // sum the copies into the accumulator on this epoch's stream
+    for (const auto& pt : O.parts)
+      for (std::unique_ptr<Request>& qp : s->active) {
+        Request& q = *qp;
+        if (q.id != pt.first || q.phase != 1 || !q.ex->fhd_acc.p || q.since_acc < c->run_photons ||
+            q.done >= q.r.photons)
+          continue;
+        bool in_flight = false;
+        for (int slot : s->inflight)
+          for (const auto& op : s->ep[slot].parts) in_flight |= op.first == q.id;
+        if (in_flight) continue;
+        if (accumulate_tallies(*q.m, q.ex.get(), O.stream) != 0) return MCFL_ERR_DEVICE;
+        if (!q.acc_ev) MCFL_HIPCHK(hipEventCreateWithFlags(&q.acc_ev, hipEventDisableTiming));
+        MCFL_HIPCHK(hipEventRecord(q.acc_ev, O.stream));
+        q.acc_wait = true;
+        q.since_acc = 0;
+      }
     for (std::unique_ptr<Request>& q : s->active)
       if ((q->phase == 1 || q->phase == 2) && !q->pending && q->finished >= q->r.photons) {
         const auto ta = std::chrono::steady_clock::now();

@@ -721,6 +721,36 @@ extern "C" uint64_t mcfl_DoOneSimulation(mcfl_ctx* c, mcfl_sim* s, uint64_t* x, 
   return fin;
 }
 
+extern "C" uint64_t mcfl_DoOneSimulationFl(mcfl_ctx* c, mcfl_sim* s, uint64_t* x, uint32_t* a, uint64_t* R_out,
+                                           uint64_t* T_out) {
+  if (!c || !s || !x || !a || !R_out || !T_out) return 0;
+  const size_t npix = npix_of(s);
+  std::vector<uint64_t> tim(s->do_temp_sim == 1 ? ntime_of(s) : 0);
+  mcfl_tallies t;
+  memset(&t, 0, sizeof t);
+  t.Rd_xy = R_out;  // det.nx + det.ny*det.nx entries, the reference's tempvoxelR/T (CUDAMCFLmain.cu:203-205,275-281)
+  t.Tt_xy = T_out;
+  t.time_xyt = tim.empty() ? nullptr : tim.data();
+  mcfl_rng rng;
+  memset(&rng, 0, sizeof rng);
+  rng.mode = MCFL_RNG_LANES;
+  rng.x = x;
+  rng.a = a;
+  rng.n_lanes = MCFL_NUM_THREADS;
+  // the caller's simulation as it stands: the reference's voxel loop has set
+  // mua <- muaf, dir = 0, fhd off and the source at the voxel corner
+  // (CUDAMCFLmain.cu:453-466,481-487); fhd is not tallied here
+  mcfl_sim sv = *s;
+  sv.fhd_activated = 0;
+  const mcfl_status st = mcfl_run_excitation(c, &sv, &rng, &t);
+  if (st != MCFL_OK) {
+    fprintf(stderr, "fluorescence voxel simulation failed: %s\n", mcfl_status_string(st));
+    for (size_t i = 0; i < npix; ++i) R_out[i] = T_out[i] = 0;
+    return 0;  // the reference's failure value (CUDAMCFLmain.cu:268-272)
+  }
+  return t.num_terminated_photons;
+}
+
 extern "C" mcfl_status mcfl_fl_plan(const mcfl_sim* s, const double* Fx, uint64_t nper, uint32_t* vox, double* scale,
                                     uint64_t* n_vox, double* unit) {
   if (!s || !Fx || !n_vox || !unit || nper == 0) return MCFL_ERR_ARG;

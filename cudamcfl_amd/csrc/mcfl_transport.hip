@@ -299,7 +299,7 @@ __device__ void exit_tally(const JobDev& J, const Ph& p, bool top, bool scaled, 
       unsigned long long inc = scaled ? (unsigned long long)((double)p.w * escale + 0.5) : (unsigned long long)p.w;
       atomic_add_u64((top ? J.Rd : J.Tt) + idx, inc);
       if (J.flags & MCFL_F_SQ)
-        atomic_add_u64((top ? J.Rd2 : J.Tt2) + idx, ((unsigned long long)p.w * (unsigned long long)p.w) >> 24);
+        atomic_add_u64((top ? J.Rd2 : J.Tt2) + idx, ((unsigned long long)p.w * (unsigned long long)p.w) >> 32);
     }
   }
   if (TEMPORAL && (J.flags & MCFL_F_TEMP) && d.t_rort == (top ? 0u : 1u) && p.tof < d.t_max) {

@@ -178,7 +178,7 @@ typedef struct mcfl_tallies {
   uint64_t* fhd;      /* nx*ny*nz photon hitting density; filled when fhd_activated and non-NULL */
   uint64_t* time_xyt; /* x_temp_numdets*y_temp_numdets*temp_bins; filled when do_temp_sim and non-NULL */
   uint64_t* A_xyz;    /* extension: absorbed weight per voxel; NULL = off */
-  uint64_t* Rd2_xy;   /* extension: per-pixel sum of (w*w)>>24 for variance; NULL = off */
+  uint64_t* Rd2_xy;   /* extension: per-pixel sum of (w*w)>>32 for variance (exact below 2^32 exits per pixel); NULL = off */
   uint64_t* Tt2_xy;
   uint64_t totals[MCFL_NTOT];
   uint64_t num_terminated_photons; /* photons finished (the reference's counter) */
@@ -210,6 +210,21 @@ mcfl_status mcfl_run_excitation(mcfl_ctx* ctx, const mcfl_sim* sim, const mcfl_r
 uint64_t mcfl_DoOneSimulation(mcfl_ctx* ctx, mcfl_sim* simulation, uint64_t* x, uint32_t* a, double* Fx);
 
 /* ---- fluorescence (emission) stage ---- */
+
+/* DoOneSimulationFl, CUDAMCFLmain.cu:196-288: one emission simulation of the
+ * reference's voxel loop, for callers that keep that loop (:471-594) and
+ * call this once per voxel.  The simulation is used as the caller left it
+ * (the loop sets mua <- muaf, dir = 0, number_of_photons = photons per voxel
+ * and the source at the voxel corner, :453-487); no fhd is tallied.  Runs on
+ * the reference lanes x, a (MCFL_NUM_THREADS of them, advanced in place, the
+ * first photon of each lane replaying its launch draws as
+ * LaunchPhoton_Global does), fills R_out and T_out (det.nx*(det.ny+1) raw
+ * sums each, the reference's tempvoxelR/T) and returns the photons finished,
+ * 0 on failure (fewer photons than lanes, as the reference, or a device
+ * error).  mcfl_run_fluorescence_lanes runs the whole loop in one launch. */
+uint64_t mcfl_DoOneSimulationFl(mcfl_ctx* ctx, mcfl_sim* simulation, uint64_t* x, uint32_t* a, uint64_t* R_out,
+                                uint64_t* T_out);
+
 
 typedef struct mcfl_fl_result {
   uint64_t* FlR_raw; /* optional det.nx*(det.ny+1) fixed-point tallies */
@@ -258,12 +273,15 @@ mcfl_status mcfl_run_fluorescence_lanes(mcfl_ctx* ctx, const mcfl_sim* sim, cons
  * cudamcfl-array.sh:15).  Many simulations -- different sources, budgets and
  * media -- share one persistent launch and one photon queue; each job's
  * photons draw from streams keyed by its own seed, so every job's tallies are
- * identical to a run of that job alone.  A job's fhd / A_xyz are summed from
- * per-XCD u64 copies and saturate at 2^64 - 1; they are exact while
- * photons * F < 8 * 2^32 (F as for mcfl_run_excitation), so a job far above
- * 2^27 photons in a strongly scattering hot voxel belongs in
- * mcfl_run_excitation, which splits it.  Served requests (below) tally fhd
- * only internally, for their emission pass, with the same bound. */
+ * identical to a run of that job alone.  Jobs take at most 2^32 - 1 photons
+ * (MCFL_ERR_CAPACITY).  A batch with a job above the photon split of
+ * mcfl_run_excitation (2^27, env MCFL_RUN_PHOTONS) runs in slices of at most
+ * that many photons per job, its fhd / A_xyz summed between slices into
+ * accumulators that saturate at 2^64 - 1 and its totals summed the same way,
+ * exactly as mcfl_run_excitation splits a long run: the results equal that
+ * call's.  Served requests (below) tally fhd only internally, for their
+ * emission pass, and sum it into an accumulator after every 2^27 photons the
+ * same way. */
 
 enum { MCFL_TALLY_FHD = 1, MCFL_TALLY_SQ = 2, MCFL_TALLY_TEMP = 4, MCFL_TALLY_AXYZ = 8 };
 #define MCFL_MAX_BATCH_JOBS 64

@@ -354,7 +354,7 @@ static void exit_tally(const ctx_t* c, const orc_photon* p, int top, tl_t* t) {
         img[idx] += (uint64_t)((double)p->weight * t->exit_scale + 0.5);
       else
         img[idx] += p->weight;
-      if (img2) img2[idx] += ((uint64_t)p->weight * (uint64_t)p->weight) >> 24;
+      if (img2) img2[idx] += ((uint64_t)p->weight * (uint64_t)p->weight) >> 32;
     }
   }
   /* :379-390 / :406-417, outside the image window test */

@@ -98,7 +98,7 @@ typedef struct {
   uint64_t* fhd;  /* nx*ny*nz (if fhd_on) */
   uint64_t* axyz; /* nx*ny*nz (if axyz_on) */
   uint64_t* time; /* t_ndx*t_ndy*t_bins (if do_temp) */
-  uint64_t* Rd2;  /* per-pixel sum of (w*w)>>24 (if sq_on) */
+  uint64_t* Rd2;  /* per-pixel sum of (w*w)>>32 (if sq_on) */
   uint64_t* Tt2;
   uint64_t tot[ORC_NTOT];
 } orc_tallies;

@@ -90,7 +90,7 @@ def test_importance_sampled_emission_matches_voxel_loop(ctx):
     b.run()
     b.close()
     img = job.Rd[:100].astype(np.float64) * job.fl_unit / 0.01
-    var = job.Rd2[:100].astype(np.float64) * 2.0 ** 24 * (job.fl_unit / 0.01) ** 2
+    var = job.Rd2[:100].astype(np.float64) * 2.0 ** 32 * (job.fl_unit / 0.01) ** 2
     tot_ref, tot_is = ref["FlR"].sum(), img.sum()
     sig = np.sqrt(var.sum()) + 1e-300
     assert abs(tot_is - tot_ref) < 5 * sig + 0.02 * tot_ref, (tot_is, tot_ref, sig)

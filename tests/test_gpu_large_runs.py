@@ -12,6 +12,7 @@ import pytest
 
 import cudamcfl_amd as M
 import _oracle as O
+from cudamcfl_amd.media import inclusion_simulation
 from media import small_sim
 
 pytestmark = pytest.mark.gpu
@@ -154,3 +155,69 @@ def test_photon_ranges_add_up(ctx, monkeypatch):
     t.Rd_xy, t.Tt_xy = Rd.ctypes.data_as(M.U64P), Rd.ctypes.data_as(M.U64P)
     sim = small_sim(photons=5000)
     assert M.lib().mcfl_run_excitation(ctx.h, C.byref(sim.s), C.byref(rng), C.byref(t)) == 1  # MCFL_ERR_ARG
+
+
+def test_split_run_on_the_bench_grid_equals_one_launch(ctx, monkeypatch):
+    """The 128x128x64 bench medium at 4e6 photons run as four launches of 1e6
+    (the per-launch copies summed into saturating accumulators, then copied
+    back on the null stream after the context's stream has finished) gives
+    the fhd, A_xyz, images and totals of one launch."""
+    sim = inclusion_simulation(photons=4_000_000)
+    a = ctx.run_excitation(sim, seed=17, axyz=True, sq=True)
+    monkeypatch.setenv("MCFL_RUN_PHOTONS", "1000000")
+    split = M.Context(0)
+    print("case: 128x128x64, 4e6 photons, one launch vs four", flush=True)
+    b = split.run_excitation(sim, seed=17, axyz=True, sq=True)
+    split.close()
+    _same(a, b)
+    assert b.fhd.sum() > 0 and b.A_xyz.sum() > 0
+
+
+def test_batch_and_server_do_not_wrap_a_hot_voxel(ctx):
+    """4e8 photons on the 3x3x1 hot-voxel medium: one launch's 8 per-XCD fhd
+    copies would each hold about 1.4 times the u64 range in the source voxel
+    and wrap.  As a batch job (beside a small one) and as a served
+    fluorescence request, the job runs in slices of 2^27 photons whose copies
+    are summed into saturating accumulators: the batch job's fhd (2^64 - 1 in
+    the source voxel), A_xyz, images and totals equal mcfl_run_excitation's
+    split run, the small job its run alone, and the served request's
+    excitation and emission images equal the split run and an emission job
+    on its fhd."""
+    N = 400_000_000
+    sim = _hot_voxel_sim(N)
+    print("case: 3x3x1 hot voxel, 4e8 photons, split run", flush=True)
+    alone = ctx.run_excitation(sim, seed=4, axyz=True)
+    centre = int(np.argmax(alone.fhd))
+    assert alone.fhd[centre] == U64_MAX and np.delete(alone.fhd, centre).max() < U64_MAX
+    jobs = [M.BatchJob(0, N, 4, tallies=M.TALLY_FHD | M.TALLY_AXYZ), M.BatchJob(0, 5000, 5, tallies=M.TALLY_FHD)]
+    print("case: 3x3x1 hot voxel, 4e8 photons, batch in slices", flush=True)
+    b = M.Batch(ctx, [sim], jobs)
+    b.run()
+    b.close()
+    j = jobs[0]
+    assert np.array_equal(j.fhd, alone.fhd) and np.array_equal(j.A_xyz, alone.A_xyz)
+    assert np.array_equal(j.Rd, alone.Rd) and np.array_equal(j.Tt, alone.Tt)
+    assert j.totals == alone.totals
+    small = ctx.run_excitation(_hot_voxel_sim(5000), seed=5)
+    assert np.array_equal(jobs[1].fhd, small.fhd) and np.array_equal(jobs[1].Rd, small.Rd)
+    assert jobs[1].totals == small.totals
+    print("case: 3x3x1 hot voxel, 4e8 photons, served fluorescence request", flush=True)
+    srv = M.Server(ctx, mem_budget=1 << 30, epoch_photons=2_000_000)
+    mid = srv.add_medium(sim)
+    rid = srv.submit(mid, N, 4, fluorescence=True)
+    done, steps = [], 0
+    while not done:
+        done, _ = srv.step()
+        steps += 1
+        assert steps < 2000
+    out = srv.collect(rid, mid)
+    srv.close()
+    assert np.array_equal(out["Rd"], alone.Rd) and np.array_equal(out["Tt"], alone.Tt)
+    assert out["totals"] == alone.totals
+    Fx = alone.fhd.astype(np.float64) / (4294967295.0 * alone.totals["photons"])
+    em = M.BatchJob(0, N, 4, src_kind=M.SOURCE_EMISSION, Fx=Fx)
+    b = M.Batch(ctx, [sim], [em])
+    b.run()
+    b.close()
+    assert np.array_equal(out["FlR"], em.Rd) and np.array_equal(out["FlT"], em.Tt)
+    assert out["fl_unit"] == em.fl_unit

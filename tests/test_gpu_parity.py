@@ -269,6 +269,45 @@ def test_fluorescence_on_reference_lanes_bit_exact(ctx):
         ctx.run_fluorescence_lanes(sim, Fx, L - 1, lanes=(x0.copy(), a))
 
 
+def test_per_voxel_emission_entry_point(ctx):
+    """mcfl_DoOneSimulationFl (CUDAMCFLmain.cu:196-288) called once per
+    voxel from a copy of the reference's voxel loop (:453-594: mua <- muaf,
+    isotropic source at the voxel corner, 32768 lanes continuing from voxel
+    to voxel): for three emitting voxels its raw R/T images, its photon count
+    and the advanced lane states equal the oracle's lane run of the same
+    voxel; fewer photons than lanes returns 0 like the reference."""
+    lab = small_labels(10, 10, 5, 4)
+    sim = M.Simulation.voxel(lab, MATS3, esp=0.5, grid_size=10, det=(0.1, 0.1, 10, 10, 0.0, 0.0), photons=40000,
+                             n_above=1.0, n_below=1.33, fhd=False)
+    s = sim.s
+    for n in range(s.n_bulks + 2):
+        s.bulks[n].mua = s.bulks[n].muaf
+    s.dir, s.fhd_activated, s.do_temp_sim = 0.0, 0, 0
+    nv = 40000
+    s.number_of_photons = nv
+    x0, a = M.init_rng(M.NUM_THREADS, seed=23)
+    xg, xo = x0.copy(), x0.copy()
+    o = O.FromProduct(sim)
+    emitting = [v for v in range(lab.size) if s.bulks[int(lab.reshape(-1)[v])].albedof < 1]
+    print("case: per-voxel emission entry point, 3 voxels of 40000 photons on 32768 lanes", flush=True)
+    for v in emitting[:3]:
+        iz, rem = divmod(v, 100)
+        iy, ix = divmod(rem, 10)
+        s.xi = np.float32(np.float32(ix) / np.float32(10)) - np.float32(s.halfx)
+        s.yi = np.float32(np.float32(iy) / np.float32(10)) - np.float32(s.halfy)
+        s.zi = np.float32(iz) / np.float32(10)
+        n, R, T = ctx.DoOneSimulationFl(sim, xg, a)
+        o.o.src_kind, o.o.xi, o.o.yi, o.o.zi = 0, s.xi, s.yi, s.zi
+        r = O.run_lanes(o.o, xo, a, nv)
+        assert n == nv == r.tot["photons"]
+        assert np.array_equal(R, r.Rd) and np.array_equal(T, r.Tt) and R.sum() > 0
+        assert np.array_equal(xg, xo)
+    assert not np.array_equal(xg, x0)
+    s.number_of_photons = M.NUM_THREADS - 1
+    n, R, T = ctx.DoOneSimulationFl(sim, xg.copy(), a)
+    assert n == 0 and R.sum() == 0
+
+
 def test_step_cap_aborts_like_the_oracle(ctx, monkeypatch):
     """The per-photon step cap (the kernel's bounded-work watchdog) forced
     down to 40 steps from the host: photons still alive after 40 steps stop

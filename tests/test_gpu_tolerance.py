@@ -20,10 +20,10 @@ def _balance(t):
 
 def _z(S1, Q1, S2, Q2, n):
     """|S1 - S2| / sigma per pixel; S: sums of w (2^-32 units), Q: sums of
-    (w*w) >> 24, so sum of w^2 = Q * 2^24 (2^-64 units)."""
+    (w*w) >> 32, so sum of w^2 = Q * 2^32 (2^-64 units)."""
     S1, S2 = S1.astype(np.float64), S2.astype(np.float64)
-    v1 = np.maximum(Q1.astype(np.float64) * 2.0 ** 24 - S1 * S1 / n, 0.0)
-    v2 = np.maximum(Q2.astype(np.float64) * 2.0 ** 24 - S2 * S2 / n, 0.0)
+    v1 = np.maximum(Q1.astype(np.float64) * 2.0 ** 32 - S1 * S1 / n, 0.0)
+    v2 = np.maximum(Q2.astype(np.float64) * 2.0 ** 32 - S2 * S2 / n, 0.0)
     return np.abs(S1 - S2) / np.sqrt(np.maximum(v1 + v2, 1.0))
 
 
