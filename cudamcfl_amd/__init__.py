@@ -92,7 +92,7 @@ _lib = None
 EXPORTS = ["mcfl_read_simulation_data", "mcfl_free_simulations", "mcfl_interpret_arg", "mcfl_init_rng",
            "mcfl_default_primes_path", "mcfl_stream_key", "mcfl_ctx_create", "mcfl_ctx_destroy",
            "mcfl_status_string", "mcfl_ctx_info", "mcfl_run_excitation", "mcfl_DoOneSimulation",
-           "mcfl_DoOneSimulationFl", "mcfl_fl_plan",
+           "mcfl_DoOneSimulationFl", "mcfl_run_excitation_noalloc", "mcfl_fl_plan",
            "mcfl_run_fluorescence", "mcfl_run_fluorescence_lanes", "mcfl_write_simulation_results", "mcfl_write_phd",
            "mcfl_write_fl"]
 
@@ -122,6 +122,7 @@ def lib():
     L.mcfl_run_excitation.argtypes = [C.c_void_p, C.POINTER(Sim), C.POINTER(Rng), C.POINTER(Tallies)]
     L.mcfl_DoOneSimulation.argtypes = [C.c_void_p, C.POINTER(Sim), U64P, U32P, F64P]
     L.mcfl_DoOneSimulation.restype = C.c_uint64
+    L.mcfl_run_excitation_noalloc.argtypes = [C.c_void_p, C.POINTER(Sim), C.POINTER(Rng), C.POINTER(Tallies)]
     L.mcfl_DoOneSimulationFl.argtypes = [C.c_void_p, C.POINTER(Sim), U64P, U32P, U64P, U64P]
     L.mcfl_DoOneSimulationFl.restype = C.c_uint64
     L.mcfl_fl_plan.argtypes = [C.POINTER(Sim), F64P, C.c_uint64, U32P, F64P, U64P, F64P]
@@ -402,9 +403,10 @@ class Context:
         check(lib().mcfl_ctx_info(self.h, C.byref(ncu), C.byref(bpc), name, 256), "mcfl_ctx_info")
         return {"n_cu": ncu.value, "blocks_per_cu": bpc.value, "arch": name.value.decode()}
 
-    def run_excitation(self, sim, seed=None, lanes=None, axyz=False, sq=False, first_photon=0):
+    def run_excitation(self, sim, seed=None, lanes=None, axyz=False, sq=False, first_photon=0, noalloc=False):
         """mcfl_run_excitation.  seed -> photon streams (photons first_photon onwards);
-        lanes=(x, a) -> reference lanes (x advanced)."""
+        lanes=(x, a) -> reference lanes (x advanced).  noalloc: the
+        allocation-free variant mcfl_run_excitation_noalloc."""
         r = ExcitationResult(sim, axyz=axyz, sq=sq)
         rng = Rng()
         if lanes is not None:
@@ -413,7 +415,8 @@ class Context:
         else:
             rng.mode, rng.seed = RNG_PHOTON, int(seed if seed is not None else 1)
         rng.first_photon = int(first_photon)  # the library refuses an offset on lane streams
-        check(lib().mcfl_run_excitation(self.h, C.byref(sim.s), C.byref(rng), C.byref(r.t)), "run_excitation")
+        fn = lib().mcfl_run_excitation_noalloc if noalloc else lib().mcfl_run_excitation
+        check(fn(self.h, C.byref(sim.s), C.byref(rng), C.byref(r.t)), "run_excitation")
         return r
 
     def DoOneSimulationFl(self, sim, x, a):

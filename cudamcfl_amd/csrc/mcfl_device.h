@@ -164,6 +164,38 @@ struct LaunchArgs {
   uint32_t max_steps;  /* per-photon step cap; 0: MCFL_MAX_STEPS_PER_PHOTON */
 };
 
+/* Scratch of the allocation-free excitation (mcfl_run_excitation_noalloc):
+   one file-scope device array per GPU (mcfl_transport.hip), cut into the
+   run's device buffers at these byte offsets (each a multiple of 256).  Its
+   limits: MCFL_NA_MAX_VOXELS labels and fhd voxels (fhd and A_xyz together:
+   half as many), MCFL_NA_MAX_PIX entries per image, MCFL_NA_MAX_TIME
+   temporal-tally entries, MCFL_NA_MAX_TDET detector positions,
+   MCFL_NA_MAX_LANES reference lanes, MCFL_MAX_MATERIALS materials. */
+#define MCFL_NA_MAX_VOXELS (1u << 20)
+#define MCFL_NA_MAX_PIX (1u << 17)
+#define MCFL_NA_MAX_TIME (1u << 16)
+#define MCFL_NA_MAX_TDET 2048u
+#define MCFL_NA_MAX_LANES 32768u
+enum : uint64_t {
+  MCFL_NA_OFF_COUNTER = 0,
+  MCFL_NA_OFF_TOT = 256,
+  MCFL_NA_OFF_JOBS = 512,   /* one JobDev, at most 1536 bytes */
+  MCFL_NA_OFF_MATS = 2048,  /* MCFL_MAX_MATERIALS MatDev entries */
+  MCFL_NA_OFF_ZB = MCFL_NA_OFF_MATS + 16384,
+  MCFL_NA_OFF_TPOS = MCFL_NA_OFF_ZB + 1024,
+  MCFL_NA_OFF_XS = MCFL_NA_OFF_TPOS + 4ull * MCFL_NA_MAX_TDET,
+  MCFL_NA_OFF_AS = MCFL_NA_OFF_XS + 8ull * MCFL_NA_MAX_LANES,
+  MCFL_NA_OFF_LABELS = MCFL_NA_OFF_AS + 4ull * MCFL_NA_MAX_LANES,
+  MCFL_NA_OFF_RD = MCFL_NA_OFF_LABELS + MCFL_NA_MAX_VOXELS,
+  MCFL_NA_OFF_TT = MCFL_NA_OFF_RD + 8ull * MCFL_NA_MAX_PIX,
+  MCFL_NA_OFF_RD2 = MCFL_NA_OFF_TT + 8ull * MCFL_NA_MAX_PIX,
+  MCFL_NA_OFF_TT2 = MCFL_NA_OFF_RD2 + 8ull * MCFL_NA_MAX_PIX,
+  MCFL_NA_OFF_TIME = MCFL_NA_OFF_TT2 + 8ull * MCFL_NA_MAX_PIX,
+  MCFL_NA_OFF_VOL = MCFL_NA_OFF_TIME + 8ull * MCFL_NA_MAX_TIME, /* per-XCD copies of fhd, then of A_xyz */
+  MCFL_NA_VOL_BYTES = 8ull * MCFL_TALLY_COPIES * MCFL_NA_MAX_VOXELS,
+  MCFL_NA_BYTES = MCFL_NA_OFF_VOL + MCFL_NA_VOL_BYTES /* 73296896 bytes, 69.9 MiB */
+};
+
 /* Dynamic LDS of a launch: per-job totals, fhd tile, material table, layer bounds. */
 /* Threads per workgroup of a launch. */
 static inline uint32_t mcfl_wg_of(const LaunchArgs* A) { return A->wg ? A->wg : (uint32_t)MCFL_WG; }

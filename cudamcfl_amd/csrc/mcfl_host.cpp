@@ -437,6 +437,8 @@ extern "C" mcfl_status mcfl_ctx_create(int device, const char* primes, mcfl_ctx*
     return st;
   };
   if (hipSetDevice(device) != hipSuccess) return fail(MCFL_ERR_DEVICE);
+  // this device's allocation-free scratch (mcfl_run_excitation_noalloc)
+  if (mcfl_scratch_address(device, &c->scratch, nullptr) != 0) return fail(MCFL_ERR_DEVICE);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MCFL_ERR_DEVICE);
   c->n_cu = prop.multiProcessorCount;
@@ -494,7 +496,55 @@ extern "C" mcfl_status mcfl_ctx_info(mcfl_ctx* c, int* n_cu, int* bpc, char* nam
   return MCFL_OK;
 }
 
-extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const mcfl_rng* rng, mcfl_tallies* out) {
+namespace {
+
+// Device buffers of one excitation run: the context's own, grown on demand,
+// or (noalloc) slices of this GPU's scratch array at the MCFL_NA_OFF_*
+// offsets, after a check against the slice's size.
+enum Slot { B_LABELS, B_ZB, B_MATS, B_JOBS, B_COUNTER, B_TOT, B_RD, B_TT, B_FHD, B_AXYZ, B_TIME, B_RD2, B_TT2,
+            B_TPOS, B_XS, B_AS, B_NSLOTS };
+
+struct RunBufs {
+  mcfl_ctx* c;
+  bool noalloc;
+  void* p[B_NSLOTS] = {};
+  uint64_t vol_used = 0;  // noalloc: bytes of the volume-copy region taken by fhd
+  RunBufs(mcfl_ctx* ctx, bool na) : c(ctx), noalloc(na) {}
+  mcfl_status get(Slot k, size_t bytes) {
+    if (!noalloc) {
+      DevBuf* own[B_NSLOTS] = {&c->labels, &c->zb, &c->mats, &c->jobs, &c->counter, &c->tot, &c->Rd, &c->Tt,
+                               &c->fhd, &c->axyz, &c->time, &c->Rd2, &c->Tt2, &c->tpos, &c->xs, &c->as};
+      STCHK(ensure(*own[k], bytes));
+      p[k] = own[k]->p;
+      return MCFL_OK;
+    }
+    static const uint64_t off[B_NSLOTS] = {MCFL_NA_OFF_LABELS, MCFL_NA_OFF_ZB, MCFL_NA_OFF_MATS, MCFL_NA_OFF_JOBS,
+                                           MCFL_NA_OFF_COUNTER, MCFL_NA_OFF_TOT, MCFL_NA_OFF_RD, MCFL_NA_OFF_TT,
+                                           MCFL_NA_OFF_VOL, MCFL_NA_OFF_VOL, MCFL_NA_OFF_TIME, MCFL_NA_OFF_RD2,
+                                           MCFL_NA_OFF_TT2, MCFL_NA_OFF_TPOS, MCFL_NA_OFF_XS, MCFL_NA_OFF_AS};
+    static const uint64_t cap[B_NSLOTS] = {MCFL_NA_MAX_VOXELS, 1024, 16384, 1536, 8, 8 * MCFL_NTOT,
+                                           8ull * MCFL_NA_MAX_PIX, 8ull * MCFL_NA_MAX_PIX, MCFL_NA_VOL_BYTES,
+                                           MCFL_NA_VOL_BYTES, 8ull * MCFL_NA_MAX_TIME, 8ull * MCFL_NA_MAX_PIX,
+                                           8ull * MCFL_NA_MAX_PIX, 4ull * MCFL_NA_MAX_TDET, 8ull * MCFL_NA_MAX_LANES,
+                                           4ull * MCFL_NA_MAX_LANES};
+    uint64_t o = off[k], limit = cap[k];
+    if (k == B_AXYZ) {  // the A_xyz copies follow the fhd copies in the volume region
+      o += vol_used;
+      limit -= vol_used;
+    }
+    if (bytes > limit) return MCFL_ERR_CAPACITY;
+    if (k == B_FHD) vol_used = (bytes + 255) & ~255ull;
+    p[k] = static_cast<char*>(c->scratch) + o;
+    return MCFL_OK;
+  }
+  template <class T>
+  T* at(Slot k) const {
+    return static_cast<T*>(p[k]);
+  }
+};
+
+// mcfl_run_excitation and its allocation-free variant: buffers from b.
+mcfl_status run_excitation(mcfl_ctx* c, const mcfl_sim* s, const mcfl_rng* rng, mcfl_tallies* out, RunBufs& b) {
   if (!c || !rng || !out || !out->Rd_xy || !out->Tt_xy) return MCFL_ERR_ARG;
   STCHK(validate(s));
   if (s->number_of_photons == 0) return MCFL_ERR_ARG;
@@ -511,28 +561,45 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
   const bool fhd_on = s->fhd_activated == 1 && out->fhd;
   const size_t nv = nvox_of(s), npix = npix_of(s);
   const size_t ntime = (size_t)s->det.x_temp_numdets * s->det.y_temp_numdets * s->det.temp_bins;
+  const uint64_t N = s->number_of_photons;
+  const bool split = !lanes && N > c->run_photons;
+  if (b.noalloc && split) return MCFL_ERR_CAPACITY;  // a split run needs accumulators beyond the scratch
 
   LaunchArgs A;
   memset(&A, 0, sizeof A);
-  STCHK(medium_upload(c, s, &A.med));
+  if (s->bulk_method == 1) {  // layered: no labels, layer bounds instead
+    std::vector<float> zb;
+    layer_bounds(s, zb);
+    STCHK(b.get(B_ZB, zb.size() * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(b.p[B_ZB], zb.data(), zb.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    fill_medium(s, nullptr, &A.med, b.at<const float>(B_ZB));
+  } else {
+    STCHK(b.get(B_LABELS, nv));
+    HIPCHK(hipMemcpyAsync(b.p[B_LABELS], s->bulk_info, nv, hipMemcpyHostToDevice, c->stream));
+    fill_medium(s, b.at<const uint8_t>(B_LABELS), &A.med, nullptr);
+  }
   std::vector<MatDev> mats;
   build_mats(s, false, mats);
-  STCHK(ensure(c->mats, mats.size() * sizeof(MatDev)));
-  HIPCHK(hipMemcpyAsync(c->mats.p, mats.data(), mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, c->stream));
+  STCHK(b.get(B_MATS, mats.size() * sizeof(MatDev)));
+  HIPCHK(hipMemcpyAsync(b.p[B_MATS], mats.data(), mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, c->stream));
 
-  STCHK(ensure(c->Rd, npix * 8));
-  STCHK(ensure(c->Tt, npix * 8));
-  STCHK(ensure(c->tot, MCFL_MAX_JOBS * MCFL_NTOT * 8));
-  STCHK(ensure(c->counter, 8));
-  HIPCHK(hipMemsetAsync(c->Rd.p, 0, npix * 8, c->stream));
-  HIPCHK(hipMemsetAsync(c->Tt.p, 0, npix * 8, c->stream));
-  HIPCHK(hipMemsetAsync(c->tot.p, 0, MCFL_NTOT * 8, c->stream));
-  HIPCHK(hipMemsetAsync(c->counter.p, 0, 8, c->stream));
+  // tallies cleared with the fill kernel
+  auto clear = [&](Slot k, size_t words) -> mcfl_status {
+    return mcfl_fill_u64(b.at<unsigned long long>(k), words, c->stream) == 0 ? MCFL_OK : MCFL_ERR_DEVICE;
+  };
+  STCHK(b.get(B_RD, npix * 8));
+  STCHK(b.get(B_TT, npix * 8));
+  STCHK(b.get(B_TOT, MCFL_NTOT * 8));
+  STCHK(b.get(B_COUNTER, 8));
+  STCHK(clear(B_RD, npix));
+  STCHK(clear(B_TT, npix));
+  STCHK(clear(B_TOT, MCFL_NTOT));
+  STCHK(clear(B_COUNTER, 1));
 
   JobDev J;
   memset(&J, 0, sizeof J);
   J.begin = 0;
-  J.count = lanes ? rng->n_lanes : s->number_of_photons;
+  J.count = lanes ? rng->n_lanes : N;
   J.k0 = lanes ? 0 : rng->first_photon;
   J.key = ::mcfl_key(rng->seed, 0);
   J.src_kind = src_kind_of(s);
@@ -543,85 +610,87 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
   J.start_weight = s->start_weight;
   J.mat_base = 0;
   J.start_lab = s->bulk_method == 1 ? start_layer(s) : 0u;
-  J.Rd = static_cast<unsigned long long*>(c->Rd.p);
-  J.Tt = static_cast<unsigned long long*>(c->Tt.p);
-  J.tot = static_cast<unsigned long long*>(c->tot.p);
+  J.Rd = b.at<unsigned long long>(B_RD);
+  J.Tt = b.at<unsigned long long>(B_TT);
+  J.tot = b.at<unsigned long long>(B_TOT);
   const uint32_t copies = c->copies;
   J.copy_stride = copies > 1 ? nv : 0;
   if (fhd_on) {
-    STCHK(ensure(c->fhd, nv * 8 * copies));
-    HIPCHK(hipMemsetAsync(c->fhd.p, 0, nv * 8 * copies, c->stream));
-    J.fhd = static_cast<unsigned long long*>(c->fhd.p);
+    STCHK(b.get(B_FHD, nv * 8 * copies));
+    STCHK(clear(B_FHD, nv * copies));
+    J.fhd = b.at<unsigned long long>(B_FHD);
     J.flags |= MCFL_F_FHD;
   }
   if (out->A_xyz) {
-    STCHK(ensure(c->axyz, nv * 8 * copies));
-    HIPCHK(hipMemsetAsync(c->axyz.p, 0, nv * 8 * copies, c->stream));
-    J.axyz = static_cast<unsigned long long*>(c->axyz.p);
+    STCHK(b.get(B_AXYZ, nv * 8 * copies));
+    STCHK(clear(B_AXYZ, nv * copies));
+    J.axyz = b.at<unsigned long long>(B_AXYZ);
     J.flags |= MCFL_F_AXYZ;
   }
   if (out->Rd2_xy && out->Tt2_xy) {
-    STCHK(ensure(c->Rd2, npix * 8));
-    STCHK(ensure(c->Tt2, npix * 8));
-    HIPCHK(hipMemsetAsync(c->Rd2.p, 0, npix * 8, c->stream));
-    HIPCHK(hipMemsetAsync(c->Tt2.p, 0, npix * 8, c->stream));
-    J.Rd2 = static_cast<unsigned long long*>(c->Rd2.p);
-    J.Tt2 = static_cast<unsigned long long*>(c->Tt2.p);
+    STCHK(b.get(B_RD2, npix * 8));
+    STCHK(b.get(B_TT2, npix * 8));
+    STCHK(clear(B_RD2, npix));
+    STCHK(clear(B_TT2, npix));
+    J.Rd2 = b.at<unsigned long long>(B_RD2);
+    J.Tt2 = b.at<unsigned long long>(B_TT2);
     J.flags |= MCFL_F_SQ;
   }
   const float* d_tpos = nullptr;
   if (temporal) {
     std::vector<float> pos;
     tdet_positions(s, pos);
-    STCHK(ensure(c->tpos, pos.size() * sizeof(float)));
-    HIPCHK(hipMemcpyAsync(c->tpos.p, pos.data(), pos.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    STCHK(ensure(c->time, ntime * 8));
-    HIPCHK(hipMemsetAsync(c->time.p, 0, ntime * 8, c->stream));
-    J.time = static_cast<unsigned long long*>(c->time.p);
+    STCHK(b.get(B_TPOS, pos.size() * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(b.p[B_TPOS], pos.data(), pos.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    STCHK(b.get(B_TIME, ntime * 8));
+    STCHK(clear(B_TIME, ntime));
+    J.time = b.at<unsigned long long>(B_TIME);
     J.flags |= MCFL_F_TEMP;
-    d_tpos = static_cast<const float*>(c->tpos.p);
+    d_tpos = b.at<const float>(B_TPOS);
   }
   fill_det(s, &J.det, d_tpos);
-  STCHK(ensure(c->jobs, sizeof(JobDev)));  // uploaded below, once its tile is placed
+  static_assert(sizeof(JobDev) <= 1536, "the scratch holds one job of at most 1536 bytes");
+  STCHK(b.get(B_JOBS, sizeof(JobDev)));  // uploaded below, once its tile is placed
 
-  A.mats = static_cast<const MatDev*>(c->mats.p);
+  A.mats = b.at<const MatDev>(B_MATS);
   A.n_mats = (uint32_t)mats.size();
-  A.counter = static_cast<unsigned long long*>(c->counter.p);
+  A.counter = b.at<unsigned long long>(B_COUNTER);
   A.a_tab = c->d_atab;
   A.a_mask = kTablePhotons - 1;
-  A.jobs = static_cast<const JobDev*>(c->jobs.p);
+  A.jobs = b.at<const JobDev>(B_JOBS);
   A.njobs = 1;
   if (lanes) {
     const size_t L = rng->n_lanes;
-    STCHK(ensure(c->xs, L * 8));
-    STCHK(ensure(c->as, L * 4));
-    HIPCHK(hipMemcpyAsync(c->xs.p, rng->x, L * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->as.p, rng->a, L * 4, hipMemcpyHostToDevice, c->stream));
+    STCHK(b.get(B_XS, L * 8));
+    STCHK(b.get(B_AS, L * 4));
+    HIPCHK(hipMemcpyAsync(b.p[B_XS], rng->x, L * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.p[B_AS], rng->a, L * 4, hipMemcpyHostToDevice, c->stream));
     A.work = MCFL_WORK_LANES;
     A.total = L;
-    A.xs = static_cast<unsigned long long*>(c->xs.p);
-    A.as = static_cast<const uint32_t*>(c->as.p);
-    A.lane_q = s->number_of_photons / L;
-    A.lane_rem = s->number_of_photons % L;
+    A.xs = b.at<unsigned long long>(B_XS);
+    A.as = b.at<const uint32_t>(B_AS);
+    A.lane_q = N / L;
+    A.lane_rem = N % L;
   } else {
     A.work = MCFL_WORK_JOBS;
-    A.total = s->number_of_photons;
+    A.total = N;
   }
   if (fhd_on) {
     choose_tile(c, A, A.med, temporal, 0);
     place_tile(A, A.med, s->xi, s->yi, J);
   }
-  const uint64_t N = s->number_of_photons;
-  if (lanes || N <= c->run_photons) {
-    HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
+  unsigned long long* fhd_out = b.at<unsigned long long>(B_FHD);
+  unsigned long long* axyz_out = b.at<unsigned long long>(B_AXYZ);
+  if (!split) {
+    HIPCHK(hipMemcpyAsync(b.p[B_JOBS], &J, sizeof J, hipMemcpyHostToDevice, c->stream));
     auto reduce = [&]() -> int {
       int e = 0;
-      if (fhd_on) e |= mcfl_reduce_copies(c->fhd.as<unsigned long long>(), nv, copies, c->stream);
-      if (out->A_xyz) e |= mcfl_reduce_copies(c->axyz.as<unsigned long long>(), nv, copies, c->stream);
+      if (fhd_on) e |= mcfl_reduce_copies(fhd_out, nv, copies, c->stream);
+      if (out->A_xyz) e |= mcfl_reduce_copies(axyz_out, nv, copies, c->stream);
       return e;
     };
     STCHK(launch_and_time(c, A, temporal, 0, &out->kernel_ms, reduce));
-    HIPCHK(hipMemcpy(out->totals, c->tot.p, MCFL_NTOT * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out->totals, b.p[B_TOT], MCFL_NTOT * 8, hipMemcpyDeviceToHost));
   } else {
     /* Photons [k0, k0 + n) per launch: photon k draws from stream k whatever
        launch runs it, so the tallies equal one launch's, except that a
@@ -630,10 +699,12 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
     if (fhd_on) {
       STCHK(ensure(c->fhd_acc, nv * 8));
       HIPCHK(hipMemsetAsync(c->fhd_acc.p, 0, nv * 8, c->stream));
+      fhd_out = c->fhd_acc.as<unsigned long long>();
     }
     if (out->A_xyz) {
       STCHK(ensure(c->axyz_acc, nv * 8));
       HIPCHK(hipMemsetAsync(c->axyz_acc.p, 0, nv * 8, c->stream));
+      axyz_out = c->axyz_acc.as<unsigned long long>();
     }
     memset(out->totals, 0, sizeof out->totals);
     out->kernel_ms = 0.0;
@@ -642,49 +713,58 @@ extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const
       J.k0 = rng->first_photon + k0;
       J.count = n;
       A.total = n;
-      HIPCHK(hipMemcpyAsync(c->jobs.p, &J, sizeof J, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemsetAsync(c->counter.p, 0, 8, c->stream));
-      HIPCHK(hipMemsetAsync(c->tot.p, 0, MCFL_NTOT * 8, c->stream));
+      HIPCHK(hipMemcpyAsync(b.p[B_JOBS], &J, sizeof J, hipMemcpyHostToDevice, c->stream));
+      STCHK(clear(B_COUNTER, 1));
+      STCHK(clear(B_TOT, MCFL_NTOT));
       auto accumulate = [&]() -> int {
         int e = 0;
-        if (fhd_on)
-          e |= mcfl_accumulate_copies(c->fhd.as<unsigned long long>(), nv, copies,
-                                      c->fhd_acc.as<unsigned long long>(), c->stream);
+        if (fhd_on) e |= mcfl_accumulate_copies(b.at<unsigned long long>(B_FHD), nv, copies, fhd_out, c->stream);
         if (out->A_xyz)
-          e |= mcfl_accumulate_copies(c->axyz.as<unsigned long long>(), nv, copies,
-                                      c->axyz_acc.as<unsigned long long>(), c->stream);
+          e |= mcfl_accumulate_copies(b.at<unsigned long long>(B_AXYZ), nv, copies, axyz_out, c->stream);
         return e;
       };
       double ms = 0.0;
       STCHK(launch_and_time(c, A, temporal, 0, &ms, accumulate));
       out->kernel_ms += ms;
       uint64_t t[MCFL_NTOT];
-      HIPCHK(hipMemcpy(t, c->tot.p, sizeof t, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(t, b.p[B_TOT], sizeof t, hipMemcpyDeviceToHost));
       for (int i = 0; i < MCFL_NTOT; ++i) {
         const uint64_t a = out->totals[i] + t[i];
         out->totals[i] = a < t[i] ? ~0ull : a;
       }
     }
-    if (fhd_on) HIPCHK(hipMemcpyAsync(c->fhd.p, c->fhd_acc.p, nv * 8, hipMemcpyDeviceToDevice, c->stream));
-    if (out->A_xyz) HIPCHK(hipMemcpyAsync(c->axyz.p, c->axyz_acc.p, nv * 8, hipMemcpyDeviceToDevice, c->stream));
   }
   // The copies below run on the null stream, which does not wait for the
-  // context's non-blocking stream: finish its work (the accumulated copies
-  // above, or the clears of a skipped launch) first.
+  // context's non-blocking stream: finish its work first.
   HIPCHK(hipStreamSynchronize(c->stream));
 
-  HIPCHK(hipMemcpy(out->Rd_xy, c->Rd.p, npix * 8, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(out->Tt_xy, c->Tt.p, npix * 8, hipMemcpyDeviceToHost));
-  if (fhd_on) HIPCHK(hipMemcpy(out->fhd, c->fhd.p, nv * 8, hipMemcpyDeviceToHost));
-  if (out->A_xyz) HIPCHK(hipMemcpy(out->A_xyz, c->axyz.p, nv * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out->Rd_xy, b.p[B_RD], npix * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out->Tt_xy, b.p[B_TT], npix * 8, hipMemcpyDeviceToHost));
+  if (fhd_on) HIPCHK(hipMemcpy(out->fhd, fhd_out, nv * 8, hipMemcpyDeviceToHost));
+  if (out->A_xyz) HIPCHK(hipMemcpy(out->A_xyz, axyz_out, nv * 8, hipMemcpyDeviceToHost));
   if (J.flags & MCFL_F_SQ) {
-    HIPCHK(hipMemcpy(out->Rd2_xy, c->Rd2.p, npix * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(out->Tt2_xy, c->Tt2.p, npix * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out->Rd2_xy, b.p[B_RD2], npix * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out->Tt2_xy, b.p[B_TT2], npix * 8, hipMemcpyDeviceToHost));
   }
-  if (temporal) HIPCHK(hipMemcpy(out->time_xyt, c->time.p, ntime * 8, hipMemcpyDeviceToHost));
-  if (lanes) HIPCHK(hipMemcpy(rng->x, c->xs.p, (size_t)rng->n_lanes * 8, hipMemcpyDeviceToHost));
+  if (temporal) HIPCHK(hipMemcpy(out->time_xyt, b.p[B_TIME], ntime * 8, hipMemcpyDeviceToHost));
+  if (lanes) HIPCHK(hipMemcpy(rng->x, b.p[B_XS], (size_t)rng->n_lanes * 8, hipMemcpyDeviceToHost));
   out->num_terminated_photons = out->totals[MCFL_TOT_PHOTONS];
   return MCFL_OK;
+}
+
+}  // namespace
+
+extern "C" mcfl_status mcfl_run_excitation(mcfl_ctx* c, const mcfl_sim* s, const mcfl_rng* rng, mcfl_tallies* out) {
+  if (!c) return MCFL_ERR_ARG;
+  RunBufs b(c, false);
+  return run_excitation(c, s, rng, out, b);
+}
+
+extern "C" mcfl_status mcfl_run_excitation_noalloc(mcfl_ctx* c, const mcfl_sim* s, const mcfl_rng* rng,
+                                                   mcfl_tallies* out) {
+  if (!c || !c->scratch) return MCFL_ERR_ARG;
+  RunBufs b(c, true);
+  return run_excitation(c, s, rng, out, b);
 }
 
 extern "C" uint64_t mcfl_DoOneSimulation(mcfl_ctx* c, mcfl_sim* s, uint64_t* x, uint32_t* a, double* Fx) {

@@ -17,6 +17,8 @@ extern "C" int mcfl_transport_occupancy(int work, int temporal, int perjob, int 
                                         int* blocks_per_cu);
 extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perjob, int nblocks, hipStream_t stream);
 extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
+// Device address of the current device's allocation-free scratch (MCFL_NA_BYTES).
+extern "C" int mcfl_scratch_address(int device, void** out, uint64_t* bytes);
 extern "C" int mcfl_reduce_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, hipStream_t stream);
 // acc[i] += the sum of the ncopies copies of element i, both saturating at
 // 2^64-1; the copies are zeroed (ready for the next launch of a chunked run).
@@ -69,6 +71,7 @@ struct mcfl_ctx {
                                         // of at most this many photons (env MCFL_RUN_PHOTONS)
   char name[256] = {};
   uint32_t* d_atab = nullptr;
+  void* scratch = nullptr;  // this device's allocation-free scratch (MCFL_NA_BYTES), looked up at creation
   std::vector<uint32_t> atab;
   mcfl_detail::DevBuf labels, zb, mats, jobs, counter, tot, Rd, Tt, fhd, axyz, time, Rd2, Tt2, tpos, xs, as, vox, vscale, fhd_acc, axyz_acc;
 };

@@ -29,6 +29,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "mcfl_device.h"
 #include "mcfl_mwc.h"
 
@@ -803,6 +805,14 @@ __global__ void mcfl_accumulate_copies_kernel(unsigned long long* p, uint64_t n,
   }
 }
 
+/* Scratch of the allocation-free excitation: the intermediate device buffers
+   of one run (work counter, totals, job and material tables, labels, images,
+   per-XCD volume-tally copies), laid out by the MCFL_NA_OFF_* offsets of
+   mcfl_device.h.  Each GPU the library runs on holds its own copy until the
+   process ends; every allocation-free call on a GPU uses that GPU's copy, so
+   such calls on one GPU must not overlap. */
+static __device__ __align__(256) unsigned long long scratch[MCFL_NA_BYTES / 8];
+
 /* ------------------------------------------------------------------ host */
 
 namespace {
@@ -907,4 +917,23 @@ extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stre
   if (blocks > 4096) blocks = 4096;
   mcfl_fill_u64_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(p, n);
   return (int)hipGetLastError();
+}
+
+/* Device address of this GPU's scratch array, looked up once per device
+   (the device must be current) and kept: the array's name is a host-side
+   stand-in and is never passed to a kernel or a copy. */
+extern "C" int mcfl_scratch_address(int device, void** out, uint64_t* bytes) {
+  static std::mutex mu;
+  static void* addr[256] = {};
+  if (!out || device < 0 || device >= 256) return -1;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return e != hipSuccess ? (int)e : -1;
+    addr[device] = p;
+  }
+  *out = addr[device];
+  if (bytes) *bytes = MCFL_NA_BYTES;
+  return 0;
 }

@@ -203,6 +203,20 @@ typedef struct mcfl_tallies {
  * N * F < 8 * 2^32 (N * F < 2^32 with one copy). */
 mcfl_status mcfl_run_excitation(mcfl_ctx* ctx, const mcfl_sim* sim, const mcfl_rng* rng, mcfl_tallies* out);
 
+/* No counterpart in the reference.  mcfl_run_excitation for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation): the same arguments, results and
+ * errors, but every device buffer of the run is a slice of one fixed scratch
+ * array per GPU (69.9 MiB, held from the library's load until the process
+ * ends), so the call allocates no device memory.  Limits, refused with
+ * MCFL_ERR_CAPACITY: at most 2^20 voxels (2^19 with both fhd and A_xyz),
+ * det.nx*(det.ny+1) <= 2^17, temporal tally <= 2^16 entries and <= 2048
+ * detectors, <= 32768 reference lanes, and no split run (photon streams of
+ * at most 2^27 photons, env MCFL_RUN_PHOTONS).  All such calls on one GPU
+ * share its scratch: calls that overlap on one GPU (two contexts or two
+ * threads) race and must be serialised by the caller. */
+mcfl_status mcfl_run_excitation_noalloc(mcfl_ctx* ctx, const mcfl_sim* sim, const mcfl_rng* rng, mcfl_tallies* out);
+
 /* DoOneSimulation, CUDAMCFLmain.cu:60-193: reference lane streams (x, a of
  * MCFL_NUM_THREADS lanes, advanced in place), writes _param/_Refl/_Trans
  * [/_time] files, fills Fx = fhd/((2^32-1) N_finished) when fhd_activated,

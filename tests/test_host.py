@@ -322,6 +322,7 @@ def test_entry_points_refuse_null_arguments():
         assert call() == ERR_ARG, name
     assert lib.mcfl_DoOneSimulation(None, S, None, None, F) == 0
     assert lib.mcfl_DoOneSimulationFl(None, S, None, None, None, None) == 0
+    assert lib.mcfl_run_excitation_noalloc(None, S, None, None) == ERR_ARG
     lib.mcfl_ctx_destroy(None)
     lib.mcfl_batch_destroy(None)
     lib.mcfl_server_destroy(None)
