@@ -1,12 +1,19 @@
-"""Benchmark of the plain transport path on one MI355X.
+"""Benchmark of the voxelized fluorescence Monte Carlo path on one MI355X.
 
-Workload: the 128x128x64-voxel seeded medium (0.5 mm voxels, 5 spheres + 2
-cylinders, 8 materials), pencil beam at the centre, 1e8 photons per step,
-photon hitting density (3D) and 128x128 reflectance/transmittance images
-tallied.  One step = one whole simulation = one persistent kernel launch.
-`value` is photons per second of device time with the medium resident in HBM
-(HIP events on the launch stream); the wall time including host<->device
-copies is reported beside it.
+Headline (`value`): the plain path.  The 128x128x64-voxel seeded medium (0.5
+mm voxels, 5 spheres + 2 cylinders, 8 materials), pencil beam at the centre,
+1e8 photons per step, photon hitting density (3D) and 128x128
+reflectance/transmittance images tallied.  One step = one whole simulation
+= one persistent kernel launch.  `value` is photons per second of device
+time with the medium resident in HBM (HIP events on the launch stream); the
+wall time including host<->device copies is reported beside it.
+
+The other half of the metric, in the same line: the 64-source scan batched
+in one launch against 64 plain launches and against the drop-in workflow of
+one `mcfl` process per .mci file, and a served Poisson request stream
+(p50/p99 latency) against a one-launch-per-request loop; each form with its
+photons/s and photon-steps/s.  `cpu_baseline`: the oracle built for the host
+(-O3 -march=native) on the host's cores.
 
 Prints one JSON line.  --gpus N>1 exits: this build runs on one GPU.
 """
@@ -21,45 +28,108 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-F_STEP = 150            # VALU issue slots per photon-step of the reference algorithm
-PEAK_SLOTS = 256 * 4 * 16 * 2.4e9   # MI355X unpacked FP32 lane-issue rate (slots/s)
+# VALU issue slots per photon-step of the reference loop (CUDAMCFLtransport.cu:
+# 286-447, counted with transcendentals and 64-bit multiply-adds at 4 slots).
+# Of these, the time-of-flight update (6) only feeds the temporal tally and
+# the fhd index and guard (11) only the photon hitting density: a launch
+# without them needs fewer.
+F_STEP_REF, F_TOF, F_FHD = 150, 6, 11
 
 
-def cpu_baseline(target_s=20.0):
-    """The CPU oracle on host cores, photon streams, same medium; a bounded
-    sample sized to about target_s seconds."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import cudamcfl_amd as M
-    import _oracle as O
-    from cudamcfl_amd.media import inclusion_simulation
+def f_step(temporal, fhd):
+    return F_STEP_REF - (0 if temporal else F_TOF) - (0 if fhd else F_FHD)
 
-    threads = min(16, os.cpu_count() or 1)
-    sim = inclusion_simulation(photons=10_000)
-    o = O.FromProduct(sim)
-    a_tab = M.multipliers(32768)
-    key = M.stream_key(99, 0)
-    n = 4000
-    for _ in range(3):  # calibrate the sample size on growing runs (thread start-up dominates tiny ones)
+
+# MI355X FP32 VALU issue rate: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz lane slots/s
+# (the published 157.3 TFLOP/s FP32 vector figure / 2 for FMA / 2 for packed
+# FP32, which this build does not use); one wave64 FP32 instruction per 4 cycles
+# per SIMD, confirmed by an FMA-chain calibration.
+PEAK_SLOTS = 256 * 4 * 16 * 2.4e9
+CPU_THREADS_MAX = 16  # the host CPU share of one GPU on the measuring boxes
+
+
+def _host_threads():
+    """Threads for the CPU baseline: the cores this process may run on, at
+    most the host's CPU share of one GPU."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return min(avail, CPU_THREADS_MAX), avail
+
+
+def _native_oracle():
+    """The oracle compiled for this host (-O3 -march=native, FP contraction
+    off, no fast-math) into a temporary directory; None if gcc fails."""
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp(prefix="mcfl_oracle_native_")
+    so = os.path.join(d, "liboracle.so")
+    cmd = ["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fno-math-errno", "-fopenmp", "-fPIC", "-std=gnu11",
+           "-shared", "-o", so, os.path.join(ROOT, "oracle", "mcfl_oracle.c"), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return (so, " ".join(cmd[:4])) if r.returncode == 0 else (None, r.stderr[-300:])
+
+
+def _timed_sample(O, sim_o, a_tab, key, threads, target_s):
+    """The oracle on growing photon counts until a run takes ~target_s."""
+    n = 2000
+    for _ in range(4):  # calibrate on growing runs (thread start-up dominates tiny ones)
         t0 = time.perf_counter()
-        r = O.run_photons(o.o, a_tab, key, 0, n, nthreads=threads)
+        O.run_photons(sim_o, a_tab, key, 0, n, nthreads=threads)
         dt = time.perf_counter() - t0
         if dt > 0.5:
             break
         n *= 8
-    n = int(max(4000, min(40_000_000, n * target_s / max(dt, 1e-3))))
+    n = int(max(2000, min(40_000_000, n * target_s / max(dt, 1e-3))))
     t0 = time.perf_counter()
-    r = O.run_photons(o.o, a_tab, key, 0, n, nthreads=threads)
-    dt = time.perf_counter() - t0
-    # one host thread, a ~3 s sample
-    n1 = max(2000, int(n / threads * 3.0 / target_s))
-    t0 = time.perf_counter()
-    r1 = O.run_photons(o.o, a_tab, key, 0, n1, nthreads=1)
-    dt1 = time.perf_counter() - t0
+    r = O.run_photons(sim_o, a_tab, key, 0, n, nthreads=threads)
+    return n, time.perf_counter() - t0, r
+
+
+def cpu_baseline(target_s=20.0):
+    """The CPU oracle (photon streams) on the host's cores, built for this host
+    at -O3 -march=native: the bench medium (bounded sample of ~target_s s) and
+    the small 80x80x20 sphere problem, each also on one thread.  The native
+    build's tallies are checked equal to the shipped build's on a small run."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cudamcfl_amd as M
+    import _oracle as O
+    from cudamcfl_amd.media import inclusion_simulation, sphere_simulation
+
+    threads, avail = _host_threads()
+    a_tab = M.multipliers(32768)
+    sim2 = inclusion_simulation(photons=10_000)
+    o2 = O.FromProduct(sim2)
+    so, build = _native_oracle()
+    same_as_shipped = None
+    if so:
+        ref = O.run_photons(o2.o, a_tab, 12345, 0, 4000, nthreads=threads)  # the shipped -O2 build
+        O._lib, O.LIB_PATH = None, so
+        nat = O.run_photons(o2.o, a_tab, 12345, 0, 4000, nthreads=threads)
+        same_as_shipped = bool(np.array_equal(ref.Rd, nat.Rd) and np.array_equal(ref.fhd, nat.fhd) and
+                               ref.tot == nat.tot)
+    else:
+        build = "shipped -O2 -march=x86-64-v3 build (native build failed: %s)" % build
+    key = M.stream_key(99, 0)
+    n, dt, r = _timed_sample(O, o2.o, a_tab, key, threads, target_s)
+    n1, dt1, r1 = _timed_sample(O, o2.o, a_tab, key, 1, 3.0)
+    sim1 = sphere_simulation(photons=10_000)
+    o1 = O.FromProduct(sim1)
+    m, dtm, rm = _timed_sample(O, o1.o, a_tab, key, threads, target_s / 2)
+    m1, dtm1, rm1 = _timed_sample(O, o1.o, a_tab, key, 1, 3.0)
     return {"value": n / dt, "unit": "photons/s", "cores": threads, "kind": "port",
-            "sample": f"{n} photons of the same 128x128x64 medium (photon streams), {dt:.1f} s",
-            "photon_steps_per_s": r.tot["steps"] / dt,
+            "sample": f"{n} photons of the bench medium (128x128x64, photon streams, fhd on), {dt:.1f} s",
+            "photon_steps_per_s": r.tot["steps"] / dt, "steps_per_photon": r.tot["steps"] / n,
+            "build": build, "native_build_equals_shipped": same_as_shipped,
+            "host_cpus_available": avail,
             "single_thread": {"photons_per_s": n1 / dt1, "photon_steps_per_s": r1.tot["steps"] / dt1,
-                              "sample": f"{n1} photons, {dt1:.1f} s"}}
+                              "sample": f"{n1} photons, {dt1:.1f} s"},
+            "small_sphere": {"workload": "80x80x20 voxels of 1 mm, sphere r 0.5 cm, pencil beam (the small "
+                                         "plain configuration)",
+                             "photons_per_s": m / dtm, "photon_steps_per_s": rm.tot["steps"] / dtm,
+                             "sample": f"{m} photons, {dtm:.1f} s, {threads} threads",
+                             "single_thread_photons_per_s": m1 / dtm1}}
 
 
 def measured_traffic(photons):
@@ -89,26 +159,25 @@ def copy_probe(nbytes):
     hip.hipFree(b)
 
 
-def rse_stats(jobs):
+def rse_stats(jobs, frac):
     """Per-pixel relative standard error of the reflectance images from the
-    sum and sum-of-squares tallies: RSE = sqrt(Q - S^2/N) / S, over pixels
-    above 1e-4 of each image's peak."""
+    sum and sum-of-squares tallies, RSE = sqrt(Q - S^2/N) / S, over the
+    pixels above `frac` of each image's peak."""
     vals = []
     for j in jobs:
         S = j.Rd.astype(np.float64)
         Q = j.Rd2.astype(np.float64) * 2.0 ** 32
-        keep = S > 1e-4 * S.max()
-        r = np.sqrt(np.maximum(Q[keep] - S[keep] ** 2 / j.photons, 0.0)) / S[keep]
-        vals.append(r)
+        keep = S > frac * S.max()
+        vals.append(np.sqrt(np.maximum(Q[keep] - S[keep] ** 2 / j.photons, 0.0)) / S[keep])
     v = np.concatenate(vals)
-    return {"median": float(np.median(v)), "p90": float(np.percentile(v, 90)),
-            "frac_le_1pct": float(np.mean(v <= 0.01)), "pixels": int(v.size)}
+    return {"above_frac_of_peak": frac, "pixels": int(v.size), "median": float(np.median(v)),
+            "p90": float(np.percentile(v, 90)), "max": float(v.max()), "frac_le_1pct": float(np.mean(v <= 0.01))}
 
 
-def bench_scan(ctx):
+def bench_scan(ctx, lanes=False):
     """64-source scan on the 128x128x64 medium: one graph-captured batched
-    launch vs 64 plain launches with the same budgets and seeds (bit-identical
-    images, hence identical per-pixel RSE)."""
+    launch (autotuned) vs 64 plain launches with the same budgets and seeds
+    (bit-identical images, hence identical per-pixel RSE)."""
     import cudamcfl_amd as M
     from cudamcfl_amd.media import inclusion_simulation, scan_sources, scan_budgets
     sim = inclusion_simulation(photons=1000, fhd=False)
@@ -124,8 +193,9 @@ def bench_scan(ctx):
     t0 = time.perf_counter()
     kms = b.run()
     wall_b = time.perf_counter() - t0
+    steps_b = sum(j.totals["steps"] for j in jobs)
     print(f"scan batched: {wall_b:.2f} s wall, {kms:.0f} ms device", file=sys.stderr, flush=True)
-    plain_wall, plain_ms, same = 0.0, 0.0, True
+    plain_wall, plain_ms, steps_p, same = 0.0, 0.0, 0, True
     for i, (s, n) in enumerate(zip(srcs, budgets)):
         sim.s.number_of_photons = int(n)
         sim.s.xi, sim.s.yi, sim.s.zi = s
@@ -133,33 +203,74 @@ def bench_scan(ctx):
         r = ctx.run_excitation(sim, seed=700 + i, sq=True)
         plain_wall += time.perf_counter() - t0
         plain_ms += r.kernel_ms
+        steps_p += r.totals["steps"]
         same &= bool(np.array_equal(r.Rd, jobs[i].Rd) and np.array_equal(r.Rd2, jobs[i].Rd2))
     print(f"scan plain: {plain_wall:.2f} s wall, {plain_ms:.0f} ms device, identical={same}", file=sys.stderr,
           flush=True)
-    # the reference's launch geometry: 32768 MWC lanes per simulation (DoOneSimulation)
-    lanes_wall, lanes_ms, lane_res = 0.0, 0.0, []
-    for i, (s, n) in enumerate(zip(srcs, budgets)):
-        sim.s.number_of_photons = int(n)
-        sim.s.xi, sim.s.yi, sim.s.zi = s
-        x, a = M.init_rng(M.NUM_THREADS, seed=900 + i)
-        t0 = time.perf_counter()
-        r = ctx.run_excitation(sim, lanes=(x, a), sq=True)
-        lanes_wall += time.perf_counter() - t0
-        lanes_ms += r.kernel_ms
-        r.photons = int(n)
-        lane_res.append(r)
-    print(f"scan plain, reference lanes: {lanes_wall:.2f} s wall, {lanes_ms:.0f} ms device", file=sys.stderr,
-          flush=True)
-    out = {"workload": "8x8 pencil scan, 8-voxel pitch, budgets log-uniform [1e6,5e7] (seed 7), 128x128x64 medium",
-           "photons": total, "batched_photons_per_s": total / wall_b, "batched_device_ms": kms,
-           "plain_photons_per_s": total / plain_wall, "plain_device_ms": plain_ms,
+    out = {"workload": "8x8 pencil scan, 8-voxel pitch, budgets log-uniform [1e6,5e7] (seed 7), 128x128x64 medium, "
+                       "R/T images and per-pixel sums of squared weights",
+           "photons": total,
+           "batched": {"photons_per_s": total / wall_b, "device_ms": kms, "wall_s": wall_b,
+                       "photon_steps_per_s": steps_b / (kms * 1e-3), "photon_steps": steps_b},
+           "plain_launches": {"photons_per_s": total / plain_wall, "device_ms": plain_ms, "wall_s": plain_wall,
+                              "photon_steps_per_s": steps_p / (plain_ms * 1e-3), "photon_steps": steps_p},
            "speedup_wall": plain_wall / wall_b, "speedup_device": plain_ms / kms,
-           "images_identical": same, "rse": rse_stats(jobs), "autotune": tune,
-           "plain_reference_lanes": {"lanes": M.NUM_THREADS, "photons_per_s": total / lanes_wall,
-                                     "device_ms": lanes_ms, "speedup_wall": lanes_wall / wall_b,
-                                     "rse": rse_stats(lane_res)}}
+           "images_identical": same, "rse": [rse_stats(jobs, 0.1), rse_stats(jobs, 0.01)], "autotune": tune}
+    if lanes:  # the reference's launch geometry: 32768 MWC lanes per simulation (DoOneSimulation)
+        lanes_wall, lanes_ms = 0.0, 0.0
+        for i, (s, n) in enumerate(zip(srcs, budgets)):
+            sim.s.number_of_photons = int(n)
+            sim.s.xi, sim.s.yi, sim.s.zi = s
+            x, a = M.init_rng(M.NUM_THREADS, seed=900 + i)
+            t0 = time.perf_counter()
+            r = ctx.run_excitation(sim, lanes=(x, a), sq=True)
+            lanes_wall += time.perf_counter() - t0
+            lanes_ms += r.kernel_ms
+        out["plain_reference_lanes"] = {"lanes": M.NUM_THREADS, "photons_per_s": total / lanes_wall,
+                                        "device_ms": lanes_ms, "wall_s": lanes_wall}
     b.close()
     return out
+
+
+def bench_dropin(n_files=64):
+    """The drop-in workflow of the 64-source scan: one `mcfl <file>.mci -S7`
+    process per source (the reference's one-process-per-file scans) against
+    one `mcfl f1.mci ... f64.mci -S7` process (the files batched), wall time
+    of the processes included; the image files are compared byte for byte."""
+    import subprocess
+    import tempfile
+    from cudamcfl_amd.media import inclusion_medium, scan_budgets, scan_sources, write_bulk, write_mci
+    exe = os.path.join(ROOT, "cudamcfl_amd", "_build", "mcfl")
+    lab, mats = inclusion_medium(2024)
+    nz, ny, nx = lab.shape
+    srcs, budgets = scan_sources()[:n_files], scan_budgets()[:n_files]
+    base = tempfile.mkdtemp(prefix="mcfl_dropin_")
+    for d in ("one_each", "batched"):
+        os.makedirs(os.path.join(base, d))
+        write_bulk(os.path.join(base, d, "medium.bulk"), lab)
+        for i, (src, n) in enumerate(zip(srcs, budgets)):
+            write_mci(os.path.join(base, d, f"s{i}.mci"), f"s{i}", "medium.bulk", mats, esp=nz / 20, grid_size=20,
+                      photons=int(n), do_fl=0, det=(0.05, 0.05, nx, ny, 0.0, 0.0), source=src, dims=(nx, ny, nz))
+    names = [f"s{i}.mci" for i in range(len(srcs))]
+    t0 = time.perf_counter()
+    for f in names:
+        r = subprocess.run([exe, f, "-S7"], cwd=os.path.join(base, "one_each"), capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout[-2000:]
+    wall_each = time.perf_counter() - t0
+    print(f"drop-in: {len(names)} processes, {wall_each:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    r = subprocess.run([exe] + names + ["-S7"], cwd=os.path.join(base, "batched"), capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    wall_one = time.perf_counter() - t0
+    print(f"drop-in: one process, {wall_one:.1f} s", file=sys.stderr, flush=True)
+    same = all(open(os.path.join(base, "one_each", f"s{i}{suf}"), "rb").read() ==
+               open(os.path.join(base, "batched", f"s{i}{suf}"), "rb").read()
+               for i in range(len(names)) for suf in ("_Refl.dat", "_Trans.dat"))
+    total = int(sum(budgets))
+    return {"workload": f"{len(names)} .mci files of the scan (128x128x64 bulk file, budgets log-uniform [1e6,5e7])",
+            "photons": total, "one_process_per_file": {"wall_s": wall_each, "photons_per_s": total / wall_each},
+            "one_process_all_files": {"wall_s": wall_one, "photons_per_s": total / wall_one},
+            "speedup_wall": wall_each / wall_one, "image_files_identical": bool(same)}
 
 
 def bench_hetero(ctx):
@@ -200,12 +311,13 @@ def bench_hetero(ctx):
 
 def _served(ctx, media, reqs, epoch):
     """Continuous batching: requests are submitted at their arrival times and
-    the server steps whenever it has work.  Returns latencies, span, counters."""
+    the server steps whenever it has work.  Returns latencies, span, photon
+    steps of the requests, server counters."""
     import cudamcfl_amd as M
     srv = M.Server(ctx, mem_budget=48 << 30, epoch_photons=epoch)
     mids = [srv.add_medium(m) for m in media]
     t0 = time.perf_counter()
-    arr, lat, i = {}, {}, 0
+    arr, lat, i, steps = {}, {}, 0, 0
     while i < len(reqs) or len(lat) < len(reqs):
         now = time.perf_counter() - t0
         while i < len(reqs) and reqs[i][0] <= now:
@@ -222,78 +334,74 @@ def _served(ctx, media, reqs, epoch):
         tnow = time.perf_counter() - t0
         for rid in done:
             lat[rid] = tnow - arr[rid][0]
-            srv.collect(rid, arr[rid][1])
+            out = srv.collect(rid, arr[rid][1])
+            steps += out["totals"]["steps"] + out["fl_totals"]["steps"]
     span = time.perf_counter() - t0
     cnt = srv.counters()
     srv.close()
-    return np.array(sorted(lat.values())), span, cnt
+    return np.array(sorted(lat.values())), span, steps, cnt
 
 
-def _plain(ctx, media, reqs, lanes=False):
-    """One launch per request (+ one emission launch) in arrival order; with
-    lanes, on the reference's 32768 per-lane MWC streams."""
+def _plain(ctx, media, reqs):
+    """One launch per request (+ one emission launch for a fluorescence
+    request) in arrival order on the same device.  Returns latencies, span,
+    photon steps and device time."""
     import cudamcfl_amd as M
     t0 = time.perf_counter()
-    lat = []
+    lat, steps, dev_ms = [], 0, 0.0
     for k, (ta, m, src, n, seed, fl) in enumerate(reqs):
         now = time.perf_counter() - t0
         if now < ta:
             time.sleep(ta - now)
         s = media[m]
-        s.s.number_of_photons = max(n, M.NUM_THREADS) if lanes else n
+        s.s.number_of_photons = n
         s.s.xi, s.s.yi, s.s.zi = src
         s.s.src_kind = M.SOURCE_PENCIL
         s.s.fhd_activated = 1 if fl else 0
-        if lanes:
-            x, a = M.init_rng(M.NUM_THREADS, seed=seed)
-            r = ctx.run_excitation(s, lanes=(x, a))
-        else:
-            r = ctx.run_excitation(s, seed=seed)
+        r = ctx.run_excitation(s, seed=seed)
+        steps += r.totals["steps"]
+        dev_ms += r.kernel_ms
         if fl:
             Fx = r.fhd.astype(np.float64) / (4294967295.0 * r.totals["photons"])
             job = M.BatchJob(0, n, seed, src_kind=M.SOURCE_EMISSION, Fx=Fx)
             bb = M.Batch(ctx, [s], [job])
-            bb.run()
+            dev_ms += bb.run()
             bb.close()
+            steps += job.totals["steps"]
         lat.append(time.perf_counter() - t0 - ta)
-        if lanes and k % 8 == 7:
-            print(f"plain, reference lanes: {k + 1} requests", file=sys.stderr, flush=True)
-    return np.array(sorted(lat)), time.perf_counter() - t0
+    return np.array(sorted(lat)), time.perf_counter() - t0, steps, dev_ms
 
 
-def bench_serve(ctx, rate=3.0, high_rate=12.0, duration=12.0, epoch=250_000):
-    """Poisson request streams: continuous batching vs a plain
-    one-launch-per-request FIFO loop on the same arrivals, at a light load
-    (also against the reference's launch geometry) and at a high load."""
+def bench_serve(ctx, rate=12.0, duration=8.0, epoch=250_000):
+    """A Poisson request stream (the served configuration's generator: media
+    up to 128x128x64, scan sources, half with fluorescence, budgets
+    log-uniform [1e5,1e8]): continuous batching against a plain
+    one-launch-per-request FIFO loop on the same arrivals and device."""
     from cudamcfl_amd.media import request_stream
 
-    def lat_stats(lat, work, span):
-        return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-                "photons_per_s": work / span, "span_s": span}
+    media, reqs = request_stream(rate=rate, duration=duration)
+    work = sum(n * (2 if fl else 1) for (_, _, _, n, _, fl) in reqs)
+    served, span_s, steps_s, cnt = _served(ctx, media, reqs, epoch)
+    print(f"served at {rate}/s: {len(served)} requests, span {span_s:.1f} s", file=sys.stderr, flush=True)
+    plain, span_p, steps_p, dev_p = _plain(ctx, media, reqs)
+    print(f"plain at {rate}/s: span {span_p:.1f} s", file=sys.stderr, flush=True)
 
-    def one(r, with_lanes):
-        media, reqs = request_stream(rate=r, duration=duration)
-        work = sum(n * (2 if fl else 1) for (_, _, _, n, _, fl) in reqs)
-        served, span_s, cnt = _served(ctx, media, reqs, epoch)
-        print(f"served at {r}/s: {len(served)} requests, span {span_s:.1f} s", file=sys.stderr, flush=True)
-        plain, span_p = _plain(ctx, media, reqs)
-        print(f"plain at {r}/s: span {span_p:.1f} s", file=sys.stderr, flush=True)
-        out = {"workload": f"Poisson arrivals {r}/s for {duration}s ({len(reqs)} requests), 8 media up to "
-                           f"128x128x64, scan sources, 50% with fluorescence, budgets log-uniform [1e5,1e8]",
-               "photons": int(work), "epoch_photons": epoch,
-               "served": dict(lat_stats(served, work, span_s), epochs=cnt["epochs"],
-                              device_photons_per_s=cnt["photons"] / max(cnt["kernel_ms"], 1e-9) * 1e3,
-                              kernel_s=cnt["kernel_ms"] / 1e3, host_s=cnt["host_ms"] / 1e3),
-               "plain": lat_stats(plain, work, span_p)}
-        if with_lanes:
-            lanes, span_l = _plain(ctx, media, reqs, lanes=True)
-            print(f"plain, reference lanes: span {span_l:.1f} s", file=sys.stderr, flush=True)
-            out["plain_reference_lanes"] = lat_stats(lanes, work, span_l)
-        return out
+    def lat(l):
+        return {"p50_s": float(np.percentile(l, 50)), "p99_s": float(np.percentile(l, 99)),
+                "mean_s": float(np.mean(l))}
 
-    out = one(rate, True)
-    out["high_load"] = one(high_rate, False)
-    return out
+    return {"workload": f"Poisson arrivals {rate}/s for {duration} s ({len(reqs)} requests), 8 media up to 128x128x64, "
+                        f"scan sources, 50% with fluorescence, budgets log-uniform [1e5,1e8] (seed 5)",
+            "photons": int(work), "epoch_photons": epoch,
+            "served": dict(lat(served), photons_per_s=work / span_s, span_s=span_s, epochs=cnt["epochs"],
+                           device_photons_per_s=cnt["photons"] / (cnt["kernel_ms"] * 1e-3),
+                           device_photon_steps_per_s=steps_s / (cnt["kernel_ms"] * 1e-3),
+                           kernel_s=cnt["kernel_ms"] / 1e3, host_s=cnt["host_ms"] / 1e3),
+            "plain_loop": dict(lat(plain), photons_per_s=work / span_p, span_s=span_p,
+                               device_photons_per_s=work / (dev_p * 1e-3),
+                               device_photon_steps_per_s=steps_p / (dev_p * 1e-3), kernel_s=dev_p / 1e3),
+            "p50_ratio_plain_over_served": float(np.percentile(plain, 50) / np.percentile(served, 50)),
+            "p99_ratio_plain_over_served": float(np.percentile(plain, 99) / np.percentile(served, 99))}
 
 
 def bench_sample2(ctx, frac=1.0):
@@ -347,17 +455,33 @@ def bench_sample2(ctx, frac=1.0):
             "total_wall_s": t2 - t0}
 
 
+def bench_small_plain(ctx, photons=1_000_000):
+    """The small plain configuration on the GPU: the reference-shaped
+    80x80x20 sphere problem, 1e6 photons, fhd and 40x40 images."""
+    from cudamcfl_amd.media import sphere_simulation
+    sim = sphere_simulation(photons=photons)
+    ctx.run_excitation(sim, seed=1)
+    r = ctx.run_excitation(sim, seed=2)
+    return {"workload": "80x80x20 voxels of 1 mm, sphere r 0.5 cm at 1 cm, pencil beam, 1e6 photons, fhd + 40x40 "
+                        "images", "photons": photons, "device_ms": r.kernel_ms,
+            "photons_per_s": photons / (r.kernel_ms * 1e-3),
+            "photon_steps_per_s": r.totals["steps"] / (r.kernel_ms * 1e-3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=1e8)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-scan", action="store_true", help="skip the 64-source batched scan")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the one-process-per-file scan")
+    ap.add_argument("--plain-only", action="store_true", help="only the plain path (for profiling)")
+    ap.add_argument("--scan-lanes", action="store_true", help="the scan also in the reference's 32768-lane geometry")
     ap.add_argument("--copy-probe", type=int, default=0, help="copy this many bytes device-to-device first")
-    ap.add_argument("--scan", action="store_true", help="also run the 64-source batched scan")
     ap.add_argument("--hetero", action="store_true", help="also run the 32-medium heterogeneous batch")
-    ap.add_argument("--serve", action="store_true", help="also run the served request stream")
     ap.add_argument("--sample2", type=float, default=0.0,
                     help="also run the reference's sample2.mci end to end, photon counts scaled by this factor")
     args = ap.parse_args()
@@ -377,7 +501,6 @@ def main():
     for w in range(args.warmup):
         ctx.run_excitation(sim, seed=1000 + w)
     kms, walls, steps = [], [], 0
-    t_all = time.perf_counter()
     for k in range(args.steps):
         t0 = time.perf_counter()
         r = ctx.run_excitation(sim, seed=k + 1)
@@ -390,11 +513,11 @@ def main():
         assert t["photons"] == N
         assert t["launched_w"] + t["roul_gain"] - t["roul_loss"] - t["zero_kill"] - t["aborted"] == \
             t["R"] + t["T"] + t["A"], "energy balance violated"
-    t_all = time.perf_counter() - t_all
     dev_s = sum(kms) / 1e3
     value = N * args.steps / dev_s
     steps_per_s = steps / dev_s
-    achieved = steps_per_s * F_STEP / 1e12
+    F = f_step(temporal=False, fhd=True)
+    achieved = steps_per_s * F / 1e12
     peak = PEAK_SLOTS / 1e12
     traffic = measured_traffic(N)
     ms_step = sum(kms) / args.steps
@@ -404,7 +527,7 @@ def main():
         gbs = traffic / (ms_step * 1e-3) / 1e9
         hbm = {"achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0}
     out = {
-        "metric": "photons/s at 128x128x64 voxels",
+        "metric": "photons/s at 128x128x64 voxels; p50/p99 sim latency, batched vs plain",
         "value": value,
         "unit": "photons/s",
         "n_gpus": 1,
@@ -425,20 +548,30 @@ def main():
         "steps_per_photon": steps / (N * args.steps),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tslot/s",
                      "frac": achieved / peak, "traffic": traffic, "hbm": hbm,
-                     "note": f"{F_STEP} VALU issue slots per photon-step x photon-steps / kernel time; traffic = "
-                             "HBM bytes per launch from FETCH_SIZE/WRITE_SIZE scaled on a copy (profiles/)"},
+                     "F_step": F, "F_step_reference": F_STEP_REF,
+                     "ceiling": "FP32 VALU issue, 256 CU x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 Tslot/s (157.3 TFLOP/s "
+                                "FP32 vector / 2 FMA / 2 packed), one wave64 FP32 instruction per 4 cycles per SIMD "
+                                "(FMA-chain calibration)",
+                     "note": f"{F} VALU issue slots per photon-step (the reference loop's {F_STEP_REF} less the "
+                             f"time-of-flight update this launch does not need: no temporal detectors) x photon-steps "
+                             "/ kernel time; traffic = HBM bytes per launch from FETCH_SIZE/WRITE_SIZE scaled on a "
+                             "copy (profiles/)"},
         "gpu": info,
     }
-    if args.scan:
-        out["batched_scan"] = bench_scan(ctx)
-    if args.hetero:
-        out["batched_hetero"] = bench_hetero(ctx)
-    if args.serve:
-        out["serving"] = bench_serve(ctx)
-    if args.sample2 > 0:
-        out["sample2_end_to_end"] = bench_sample2(ctx, args.sample2)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+    if not args.plain_only:
+        out["small_plain"] = bench_small_plain(ctx)
+        if not args.no_scan:
+            out["batched_scan"] = bench_scan(ctx, lanes=args.scan_lanes)
+        if not args.no_dropin:
+            out["dropin_scan"] = bench_dropin()
+        if not args.no_serve:
+            out["serving"] = bench_serve(ctx)
+        if args.hetero:
+            out["batched_hetero"] = bench_hetero(ctx)
+        if args.sample2 > 0:
+            out["sample2_end_to_end"] = bench_sample2(ctx, args.sample2)
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
 
 

@@ -884,9 +884,12 @@ struct mcfl_server {
   Epoch ep[2];
   int next = 0;              // slot of the next launch
   std::deque<int> inflight;  // slots in launch order
-  // cumulative counters
+  // cumulative counters; kernel_ms is the device time during which an epoch
+  // ran, overlapping epochs counted once: the union of the epochs' [start,
+  // end) intervals, measured from the event `origin` (ends at busy_end)
   uint64_t photons = 0, epochs = 0;
-  double kernel_ms = 0.0, host_ms = 0.0;
+  double kernel_ms = 0.0, host_ms = 0.0, busy_end = 0.0;
+  hipEvent_t origin = nullptr;
   // host time by part (ms): admission, epoch assembly, launch calls, phase
   // changes to emission, retirements; printed at destruction when the
   // environment sets MCFL_SERVER_PROFILE
@@ -903,6 +906,7 @@ struct mcfl_server {
       if (e.e1) (void)hipEventDestroy(e.e1);
       if (e.stream) (void)hipStreamDestroy(e.stream);
     }
+    if (origin) (void)hipEventDestroy(origin);
   }
 };
 
@@ -926,6 +930,11 @@ extern "C" mcfl_status mcfl_server_create(mcfl_ctx* c, uint64_t budget, uint64_t
       delete s;
       return MCFL_ERR_DEVICE;
     }
+  if (hipEventCreate(&s->origin) != hipSuccess || hipEventRecord(s->origin, s->ep[0].stream) != hipSuccess ||
+      hipEventSynchronize(s->origin) != hipSuccess) {
+    delete s;
+    return MCFL_ERR_DEVICE;
+  }
   *out = s;
   return MCFL_OK;
 }
@@ -1182,12 +1191,16 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
     const auto tw0 = std::chrono::steady_clock::now();
     MCFL_HIPCHK(hipEventSynchronize(O.e1));
     t_wait = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw0).count();
-    float ms = 0.f;
+    float ms = 0.f, t_start = 0.f, t_end = 0.f;
     MCFL_HIPCHK(hipEventElapsedTime(&ms, O.e0, O.e1));
+    MCFL_HIPCHK(hipEventElapsedTime(&t_start, s->origin, O.e0));
+    MCFL_HIPCHK(hipEventElapsedTime(&t_end, s->origin, O.e1));
     if (kernel_ms) *kernel_ms = ms;
     s->photons += O.photons;
     s->epochs += 1;
-    s->kernel_ms += ms;
+    // epochs complete in launch order; only the part after the busy time so far counts
+    s->kernel_ms += std::max(0.0, (double)t_end - std::max((double)t_start, s->busy_end));
+    s->busy_end = std::max(s->busy_end, (double)t_end);
     O.busy = false;
     for (const auto& pt : O.parts)
       for (std::unique_ptr<Request>& qp : s->active)

@@ -204,11 +204,13 @@ int run_many(mcfl_ctx* ctx, std::vector<mcfl_sim*>& sims, const std::vector<std:
   const int n = (int)sims.size();
   std::vector<int> single;
   std::vector<int> todo;
-  // files above the per-launch photon split of mcfl_run_excitation run alone
-  // (split into launches there); the rest share batched launches
-  const uint64_t kBatchPhotons = 1ull << 27;
+  // voxel media share batched launches (a batch splits a file above the
+  // photon split of mcfl_run_excitation the same way, so the results are a
+  // single run's); layered media, lane streams and files above a batch job's
+  // 2^32 - 1 photons run alone
   for (int i = 0; i < n; ++i)
-    (sims[i]->bulk_method == 2 && !o.lanes && sims[i]->number_of_photons <= kBatchPhotons ? todo : single).push_back(i);
+    (sims[i]->bulk_method == 2 && !o.lanes && sims[i]->number_of_photons <= 0xFFFFFFFFull ? todo : single)
+        .push_back(i);
   int rc = 0;
   size_t next = 0;
   while (next < todo.size()) {

@@ -405,7 +405,9 @@ mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32_t max_done
 /* Copies a finished request's results and forgets it. */
 mcfl_status mcfl_server_collect(mcfl_server* s, int64_t request_id, mcfl_request_result* out);
 mcfl_status mcfl_server_stats(mcfl_server* s, int32_t* queued, int32_t* active, uint64_t* bytes_in_use);
-/* Cumulative photons launched, epochs, kernel time and host time spent in steps outside the kernel. */
+/* Cumulative photons launched, epochs, device time during which an epoch ran
+   (two epochs overlapping in time count once) and host time spent in steps
+   outside the waits for the device. */
 mcfl_status mcfl_server_counters(mcfl_server* s, uint64_t* photons, uint64_t* epochs, double* kernel_ms, double* host_ms);
 void mcfl_server_destroy(mcfl_server* s);
 
