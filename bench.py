@@ -440,15 +440,20 @@ def bench_sample2(ctx, frac=1.0):
     k = job.fl_unit / (float(np.float32(s.det.dx)) * float(np.float32(s.det.dy)))
     is_tot = float(job.Rd[:xy].astype(np.float64).sum()) * k
     is_sig = float(np.sqrt(job.Rd2[:xy].astype(np.float64).sum() * 2.0 ** 32)) * k
+    ex_sps = ex.totals["steps"] / (ex.kernel_ms * 1e-3)
+    fl_sps = fl["totals"]["steps"] / (fl["kernel_ms"] * 1e-3)
+    F_ex, F_fl = f_step(temporal=True, fhd=True), f_step(temporal=False, fhd=False)
     return {"workload": f"sample2.mci ({s.nx}x{s.ny}x{s.nz} voxels from 3dbulk-sample, temporal detectors), "
                         f"photon counts x{frac:g}",
             "excitation": {"photons": n_ex, "device_ms": ex.kernel_ms, "wall_s": t1 - t0,
                            "photons_per_s": n_ex / ex.kernel_ms * 1e3,
-                           "steps_per_photon": ex.totals["steps"] / n_ex},
+                           "steps_per_photon": ex.totals["steps"] / n_ex, "photon_steps_per_s": ex_sps,
+                           "F_step": F_ex, "valu_frac": ex_sps * F_ex / PEAK_SLOTS},
             "fluorescence": {"voxels": fl["voxels"], "photons_per_voxel": n_v, "photons": fl["photons"],
                              "device_ms": fl["kernel_ms"], "wall_s": t2 - t1,
                              "photons_per_s": fl["photons"] / fl["kernel_ms"] * 1e3,
-                             "steps_per_photon": fl["totals"]["steps"] / max(1, fl["photons"])},
+                             "steps_per_photon": fl["totals"]["steps"] / max(1, fl["photons"]),
+                             "photon_steps_per_s": fl_sps, "F_step": F_fl, "valu_frac": fl_sps * F_fl / PEAK_SLOTS},
             "fluorescence_importance_sampled": {"photons": n_is, "device_ms": kms_is, "wall_s": t4 - t3,
                                                 "FlR_total": is_tot, "FlR_total_sigma": is_sig,
                                                 "FlR_total_every_voxel": float(fl["FlR"].sum())},
@@ -478,7 +483,8 @@ def main():
     ap.add_argument("--no-scan", action="store_true", help="skip the 64-source batched scan")
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
     ap.add_argument("--no-dropin", action="store_true", help="skip the one-process-per-file scan")
-    ap.add_argument("--plain-only", action="store_true", help="only the plain path (for profiling)")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="the plain path alone (with --hetero / --sample2 if given); for profiling")
     ap.add_argument("--scan-lanes", action="store_true", help="the scan also in the reference's 32768-lane geometry")
     ap.add_argument("--copy-probe", type=int, default=0, help="copy this many bytes device-to-device first")
     ap.add_argument("--hetero", action="store_true", help="also run the 32-medium heterogeneous batch")
@@ -566,12 +572,12 @@ def main():
             out["dropin_scan"] = bench_dropin()
         if not args.no_serve:
             out["serving"] = bench_serve(ctx)
-        if args.hetero:
-            out["batched_hetero"] = bench_hetero(ctx)
-        if args.sample2 > 0:
-            out["sample2_end_to_end"] = bench_sample2(ctx, args.sample2)
-        if not args.no_cpu:
-            out["cpu_baseline"] = cpu_baseline()
+    if args.hetero:
+        out["batched_hetero"] = bench_hetero(ctx)
+    if args.sample2 > 0:
+        out["sample2_end_to_end"] = bench_sample2(ctx, args.sample2)
+    if not (args.plain_only or args.no_cpu):
+        out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
 
 
