@@ -29,35 +29,62 @@ using namespace mcfl_detail;
 
 namespace {
 
-// A medium resident in device memory.
+// A medium resident in device memory: voxel labels (bulk method 2) or the
+// layer bounds of a layered slab with its inclusion (bulk method 1).
 struct Medium {
   mcfl_sim sim{};                 // scalar fields (pointers below)
   std::vector<mcfl_bulk> bulks;
+  std::vector<mcfl_layer> layers;
   std::vector<uint8_t> labels;
   std::vector<MatDev> mat_ex, mat_em;
-  DevBuf d_labels, d_tpos;
+  DevBuf d_labels, d_tpos, d_zb;
   size_t nvox = 0, npix = 0;
+  bool layered() const { return sim.bulk_method == 1; }
   ~Medium() {
     release(d_labels);
     release(d_tpos);
+    release(d_zb);
   }
 };
 
-mcfl_status medium_make(const mcfl_sim* s, Medium* m) {
+// The kernel's view of a resident medium.
+void medium_dev(const Medium& m, MediumDev* md) {
+  fill_medium(&m.sim, m.d_labels.as<const uint8_t>(), md, m.d_zb.as<const float>());
+}
+
+// The layer holding depth zi of a layered medium (0 when none does).
+uint32_t layer_at(const Medium& m, float zi) {
+  mcfl_sim t = m.sim;
+  t.zi = zi;
+  return start_layer(&t);
+}
+
+mcfl_status medium_make(const mcfl_sim* s, Medium* m, bool allow_layered) {
   MCFL_STCHK(validate(s));
-  if (s->bulk_method != 2) return MCFL_ERR_UNSUPPORTED;  // batches and serving take voxel media
+  if (s->bulk_method == 1 && !allow_layered) return MCFL_ERR_UNSUPPORTED;  // serving takes voxel media
   m->sim = *s;
   m->nvox = nvox_of(s);
   m->npix = npix_of(s);
-  m->bulks.assign(s->bulks, s->bulks + s->n_bulks + 2);
-  m->labels.assign(s->bulk_info, s->bulk_info + m->nvox);
-  m->sim.bulks = m->bulks.data();
-  m->sim.bulk_info = m->labels.data();
+  m->sim.bulks = nullptr;
+  m->sim.bulk_info = nullptr;
   m->sim.layers = nullptr;
+  if (s->bulk_method == 1) {
+    m->layers.assign(s->layers, s->layers + s->n_layers + 2);
+    m->sim.layers = m->layers.data();
+    std::vector<float> zb;
+    layer_bounds(&m->sim, zb);
+    MCFL_STCHK(ensure(m->d_zb, zb.size() * sizeof(float)));
+    MCFL_HIPCHK(hipMemcpy(m->d_zb.p, zb.data(), zb.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    m->bulks.assign(s->bulks, s->bulks + s->n_bulks + 2);
+    m->labels.assign(s->bulk_info, s->bulk_info + m->nvox);
+    m->sim.bulks = m->bulks.data();
+    m->sim.bulk_info = m->labels.data();
+    MCFL_STCHK(ensure(m->d_labels, m->nvox));
+    MCFL_HIPCHK(hipMemcpy(m->d_labels.p, m->labels.data(), m->nvox, hipMemcpyHostToDevice));
+  }
   build_mats(&m->sim, false, m->mat_ex);
   build_mats(&m->sim, true, m->mat_em);
-  MCFL_STCHK(ensure(m->d_labels, m->nvox));
-  MCFL_HIPCHK(hipMemcpy(m->d_labels.p, m->labels.data(), m->nvox, hipMemcpyHostToDevice));
   if (s->do_temp_sim == 1) {
     std::vector<float> pos;
     tdet_positions(s, pos);
@@ -354,8 +381,9 @@ void fill_job(JobDev& J, const Medium& m, Tallies& t, int src_kind, float xi, fl
   J.Tt2 = t.Tt2.as<unsigned long long>();
   J.tot = t.tot.as<unsigned long long>();
   J.copy_stride = t.copies > 1 ? m.nvox : 0;
+  if (m.layered()) J.start_lab = layer_at(m, zi);
   fill_det(&m.sim, &J.det, m.d_tpos.as<const float>());
-  fill_medium(&m.sim, m.d_labels.as<const uint8_t>(), &J.med);
+  medium_dev(m, &J.med);
   J.al_prob = t.al_prob.as<const uint32_t>();
   J.al_alias = t.al_alias.as<const uint32_t>();
   J.al_vox = t.al_vox.as<const uint32_t>();
@@ -412,7 +440,7 @@ struct Launcher {
     MCFL_STCHK(ensure(d_counter, 8 * MCFL_MAX_JOBS));
     MCFL_HIPCHK(hipMemcpyAsync(d_mats.p, mt.mats.data(), mt.mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, st));
     memset(A, 0, sizeof *A);
-    if (!perjob) fill_medium(&uniform->sim, uniform->d_labels.as<const uint8_t>(), &A->med);
+    if (!perjob) medium_dev(*uniform, &A->med);
     A->mats = d_mats.as<const MatDev>();
     A->n_mats = (uint32_t)mt.mats.size();
     A->work = MCFL_WORK_JOBS;
@@ -458,6 +486,7 @@ struct mcfl_batch {
   // between slices and the totals on the host, as mcfl_run_excitation
   // splits a long run.
   bool sliced = false;
+  bool layered = false;  // one layered medium: the 768-thread kernel only
 };
 
 namespace {
@@ -499,7 +528,7 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
   b->c = c;
   for (int i = 0; i < n_media; ++i) {
     b->media.emplace_back(new Medium());
-    MCFL_STCHK(medium_make(media[i], b->media.back().get()));
+    MCFL_STCHK(medium_make(media[i], b->media.back().get(), true));
   }
   uint64_t begin = 0;
   std::vector<bool> used(n_media, false);
@@ -511,9 +540,15 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
     if (s.photons > c->run_photons) b->sliced = true;
     const Medium& m = *b->media[s.medium];
     used[s.medium] = true;
+    const bool em = s.src_kind == MCFL_SOURCE_EMISSION;
+    if (m.layered()) {
+      // a layered slab emits nothing (emitters need albedof < 0, CUDAMCFLmain.cu:520,535), and a source
+      // outside every layer has no starting layer (the reference leaves it undefined)
+      if (em) return MCFL_ERR_UNSUPPORTED;
+      if (layer_at(m, s.zi) == 0) return MCFL_ERR_ARG;
+    }
     b->tal.emplace_back(new Tallies());
     Tallies& t = *b->tal.back();
-    const bool em = s.src_kind == MCFL_SOURCE_EMISSION;
     MCFL_STCHK(alloc_tallies(m, s.tallies, em, c->copies, &t));
     if (em) MCFL_STCHK(build_emission(m, s.Fx, s.photons, &t));
     if (t.flags & MCFL_F_TEMP) b->temporal = 1;
@@ -532,6 +567,9 @@ extern "C" mcfl_status mcfl_batch_create(mcfl_ctx* c, const mcfl_sim* const* med
       uni = b->media[i].get();
     }
   b->perjob = n_used > 1;
+  for (int i = 0; i < n_media; ++i)  // the layered kernel runs one medium per launch
+    if (used[i] && b->media[i]->layered() && b->perjob) return MCFL_ERR_UNSUPPORTED;
+  b->layered = uni->layered();
   MCFL_STCHK(b->L.prepare(c, b->jd, b->mt, b->perjob, uni, &b->A, c->stream));
   plan_tiles(c, b->A, b->jd.data(), b->jd.size(), b->temporal, b->perjob);  // an LDS fhd tile per fhd job
   MCFL_STCHK(b->L.upload(b->jd, c->stream));
@@ -570,6 +608,7 @@ extern "C" mcfl_status mcfl_batch_configure(mcfl_batch* b, const mcfl_tune* in) 
   if (!b || !in) return MCFL_ERR_ARG;
   const uint32_t wg = in->wg ? in->wg : (uint32_t)MCFL_WG;
   if (wg != MCFL_WG && wg != MCFL_WG_ALT1 && wg != MCFL_WG_ALT2) return MCFL_ERR_ARG;
+  if (b->layered && wg != MCFL_WG) return MCFL_ERR_UNSUPPORTED;
   if (in->tile_side < 0 || in->refill_min < 1 || in->refill_min > 64 || in->chunk_max < 64 || in->chunk_max > (1u << 20))
     return MCFL_ERR_ARG;
   if (hipSetDevice(b->c->device) != hipSuccess) return MCFL_ERR_DEVICE;
@@ -654,7 +693,9 @@ extern "C" mcfl_status mcfl_batch_autotune(mcfl_batch* b, double fraction, mcfl_
   const Cand dc = bc;
   MCFL_STCHK(consider(dc));
   dflt = best;
-  for (uint32_t wg : {(uint32_t)MCFL_WG_ALT1, (uint32_t)MCFL_WG_ALT2}) MCFL_STCHK(consider({wg, dc.side, dc.refill, dc.chunk}));
+  if (!b->layered)
+    for (uint32_t wg : {(uint32_t)MCFL_WG_ALT1, (uint32_t)MCFL_WG_ALT2})
+      MCFL_STCHK(consider({wg, dc.side, dc.refill, dc.chunk}));
   // stage 2: tile side x refill threshold x chunk cap at that workgroup size
   const uint32_t wg = bc.wg;
   const int sides_fhd[] = {0, 16, 32};
@@ -944,7 +985,7 @@ extern "C" mcfl_status mcfl_server_add_medium(mcfl_server* s, const mcfl_sim* si
   if (hipSetDevice(s->c->device) != hipSuccess) return MCFL_ERR_DEVICE;
   std::unique_ptr<Medium> m(new (std::nothrow) Medium());
   if (!m) return MCFL_ERR_NOMEM;
-  MCFL_STCHK(medium_make(sim, m.get()));
+  MCFL_STCHK(medium_make(sim, m.get(), false));
   if (m->mat_ex.size() > MCFL_MAX_MATERIALS / 2) return MCFL_ERR_CAPACITY;
   s->media.push_back(std::move(m));
   *id = (int32_t)s->media.size() - 1;

@@ -187,13 +187,17 @@ int run_one(mcfl_ctx* ctx, mcfl_sim* s, const char* primes, const Options& o, cl
   return 0;
 }
 
-// Same device medium: grid, labels, materials, detector and source weight.
+// Same device medium: grid, labels or layers, materials, detector and source weight.
 bool same_medium(const mcfl_sim* a, const mcfl_sim* b) {
-  if (a->nx != b->nx || a->ny != b->ny || a->nz != b->nz || a->grid_size != b->grid_size || a->esp != b->esp ||
-      a->halfx != b->halfx || a->halfy != b->halfy || a->n_bulks != b->n_bulks || a->do_temp_sim != b->do_temp_sim ||
+  if (a->bulk_method != b->bulk_method || a->nx != b->nx || a->ny != b->ny || a->nz != b->nz ||
+      a->grid_size != b->grid_size || a->esp != b->esp || a->halfx != b->halfx || a->halfy != b->halfy ||
+      a->n_bulks != b->n_bulks || a->n_layers != b->n_layers || a->do_temp_sim != b->do_temp_sim ||
       a->start_weight != b->start_weight || a->dir != b->dir)
     return false;
   if (memcmp(&a->det, &b->det, sizeof a->det) != 0) return false;
+  if (a->bulk_method == 1)  // layered slab with one inclusion
+    return memcmp(a->layers, b->layers, sizeof(mcfl_layer) * (a->n_layers + 2)) == 0 &&
+           memcmp(&a->inclusion, &b->inclusion, sizeof a->inclusion) == 0;
   if (memcmp(a->bulks, b->bulks, sizeof(mcfl_bulk) * (a->n_bulks + 2)) != 0) return false;
   return memcmp(a->bulk_info, b->bulk_info, nvox(a)) == 0;
 }
@@ -204,13 +208,13 @@ int run_many(mcfl_ctx* ctx, std::vector<mcfl_sim*>& sims, const std::vector<std:
   const int n = (int)sims.size();
   std::vector<int> single;
   std::vector<int> todo;
-  // voxel media share batched launches (a batch splits a file above the
-  // photon split of mcfl_run_excitation the same way, so the results are a
-  // single run's); layered media, lane streams and files above a batch job's
-  // 2^32 - 1 photons run alone
+  // files share batched launches (a batch splits a file above the photon
+  // split of mcfl_run_excitation the same way, so the results are a single
+  // run's; a layered medium shares a launch only with files on the same
+  // medium); lane streams and files above a batch job's 2^32 - 1 photons
+  // run alone
   for (int i = 0; i < n; ++i)
-    (sims[i]->bulk_method == 2 && !o.lanes && sims[i]->number_of_photons <= 0xFFFFFFFFull ? todo : single)
-        .push_back(i);
+    (!o.lanes && sims[i]->number_of_photons <= 0xFFFFFFFFull ? todo : single).push_back(i);
   int rc = 0;
   size_t next = 0;
   while (next < todo.size()) {
@@ -224,9 +228,10 @@ int run_many(mcfl_ctx* ctx, std::vector<mcfl_sim*>& sims, const std::vector<std:
       for (size_t k = 0; k < media.size(); ++k)
         if (same_medium(media[k], s)) m = (int)k;
       if (m < 0) {
-        if (mats + (int)s->n_bulks + 2 > 320 && !media.empty()) break;
+        const int n_mat = s->bulk_method == 1 ? (int)s->n_layers + 3 : (int)s->n_bulks + 2;
+        if (!media.empty() && (mats + n_mat > 320 || s->bulk_method == 1 || media[0]->bulk_method == 1)) break;
         media.push_back(s);
-        mats += (int)s->n_bulks + 2;
+        mats += n_mat;
         m = (int)media.size() - 1;
       }
       medium_of.push_back(m);

@@ -287,7 +287,10 @@ mcfl_status mcfl_run_fluorescence_lanes(mcfl_ctx* ctx, const mcfl_sim* sim, cons
  * cudamcfl-array.sh:15).  Many simulations -- different sources, budgets and
  * media -- share one persistent launch and one photon queue; each job's
  * photons draw from streams keyed by its own seed, so every job's tallies are
- * identical to a run of that job alone.  Jobs take at most 2^32 - 1 photons
+ * identical to a run of that job alone.  Media are voxel media (bulk method
+ * 2) or one layered slab (bulk method 1) per batch, which emits no
+ * fluorescence (MCFL_ERR_UNSUPPORTED for a layered medium beside another
+ * medium or under an emission job).  Jobs take at most 2^32 - 1 photons
  * (MCFL_ERR_CAPACITY).  A batch with a job above the photon split of
  * mcfl_run_excitation (2^27, env MCFL_RUN_PHOTONS) runs in slices of at most
  * that many photons per job, its fhd / A_xyz summed between slices into
@@ -395,7 +398,8 @@ typedef struct mcfl_request_result {
    streams; an epoch leaves 8 workgroup slots free (env MCFL_SERVER_RESERVE)
    so that finished requests' tallies are copied out beside it. */
 mcfl_status mcfl_server_create(mcfl_ctx* ctx, uint64_t mem_budget_bytes, uint64_t epoch_photons, mcfl_server** out);
-/* Makes a medium resident; the simulation's bulks/labels/detector are copied. */
+/* Makes a voxel medium resident (a layered one is MCFL_ERR_UNSUPPORTED); the
+   simulation's bulks/labels/detector are copied. */
 mcfl_status mcfl_server_add_medium(mcfl_server* s, const mcfl_sim* medium, int32_t* medium_id);
 /* Queues a request (admitted at the next step that has room). */
 mcfl_status mcfl_server_submit(mcfl_server* s, const mcfl_request* req, int64_t* request_id);

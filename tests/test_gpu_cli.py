@@ -167,3 +167,34 @@ def test_cli_lanes_on_the_layered_sample(tmp_path):
     assert done == 60000
     for suffix in ("_Refl.dat", "_Trans.dat"):
         assert (tmp_path / "cli" / ("sample1-out" + suffix)).read_bytes() == (api / ("sample1-out" + suffix)).read_bytes()
+
+
+def test_cli_layered_files_in_one_batch(tmp_path):
+    """`mcfl f1.mci f2.mci f3.mci -S5` on the reference's method-1 sample.mci
+    with three source positions (photon budget reduced) runs the three
+    excitations in one batched launch and writes, for every file, the image,
+    time and PHD files of the same file run alone."""
+    text = (GOLDEN / "sample.mci").read_text()
+    lines = text.splitlines(keepends=True)
+    src = next(i for i, l in enumerate(lines) if "ource" in l and not l.lstrip().startswith("#"))
+    assert lines[src].startswith("-4 0 0"), lines[src]
+    for d in ("many", "one"):
+        (tmp_path / d).mkdir()
+        for k, pos in enumerate(("-4 0 0", "-3.9 0.1 0", "-4.1 -0.05 0")):
+            body = lines[:]
+            body[src] = pos + " # Source position\n"
+            t = "".join(body).replace("500000000", "30000", 1).replace("sample1-out", f"f{k}")
+            (tmp_path / d / f"f{k}.mci").write_text(t)
+    print("case: mcfl CLI, 3 method-1 files in one batch vs alone, -S5", flush=True)
+    r = subprocess.run([str(EXE), "f0.mci", "f1.mci", "f2.mci", "-S5", "-F9"], cwd=tmp_path / "many",
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "in one batch" in r.stdout
+    for k in range(3):
+        r1 = subprocess.run([str(EXE), f"f{k}.mci", "-S5", "-F9"], cwd=tmp_path / "one", capture_output=True,
+                            text=True, timeout=300)
+        assert r1.returncode == 0, r1.stdout[-3000:] + r1.stderr[-3000:]
+        for suffix in ("_Refl.dat", "_Trans.dat", "_time.dat", "_PHD-Ascii.dat"):
+            a = (tmp_path / "many" / f"f{k}{suffix}").read_bytes()
+            b = (tmp_path / "one" / f"f{k}{suffix}").read_bytes()
+            assert a == b, (k, suffix)

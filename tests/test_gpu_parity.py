@@ -154,10 +154,42 @@ def test_layered_sample_mci_lanes_bit_exact(ctx):
     assert np.array_equal(x, x_cpu), "advanced lane states differ"
 
 
-def test_layered_batch_is_refused(ctx):
+def test_layered_batch_equals_single_runs(ctx):
+    """Jobs on one layered medium (bulk method 1: three layers with glass, an
+    index step, a straddling inclusion, temporal detectors) share one batched
+    launch: every job's tallies equal its plain run alone, the starting layer
+    taken from each job's own source depth; one job equals the oracle.  A
+    layered medium next to another medium in one batch, an emission job on a
+    layered medium (method 1 emits nothing) and a source outside every layer
+    are refused."""
     sim = layered3(1000)
+    srcs = [(0.0, 0.0, 0.0), (0.12, -0.05, 0.0), (-0.1, 0.08, 0.03)]
+    budgets = [6000, 2500, 4000]
+    tallies = M.TALLY_FHD | M.TALLY_SQ | M.TALLY_TEMP
+    jobs = [M.BatchJob(0, n, 40 + i, source=src, tallies=tallies) for i, (src, n) in enumerate(zip(srcs, budgets))]
+    print("case: batch of 3 sources on a layered medium", flush=True)
+    b = M.Batch(ctx, [sim], jobs)
+    b.run()
+    b.close()
+    for i, (src, n) in enumerate(zip(srcs, budgets)):
+        sim.s.number_of_photons = n
+        sim.s.xi, sim.s.yi, sim.s.zi = src
+        r = ctx.run_excitation(sim, seed=40 + i, sq=True)
+        j = jobs[i]
+        for name in ("Rd", "Tt", "fhd", "time", "Rd2"):
+            assert np.array_equal(getattr(j, name), getattr(r, name)), (i, name)
+        assert j.totals == r.totals, i
+    sim.s.xi, sim.s.yi, sim.s.zi = srcs[2]
+    o = O.FromProduct(sim, sq=True)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(42, 0), 0, budgets[2])
+    assert np.array_equal(jobs[2].Rd, r.Rd) and np.array_equal(jobs[2].fhd, r.fhd)
+    check_totals(jobs[2].totals, r.tot)
     with pytest.raises(M.McflError):
-        M.Batch(ctx, [sim], [M.BatchJob(0, 1000, 1)])
+        M.Batch(ctx, [sim, small_sim(1000)], [M.BatchJob(0, 1000, 1), M.BatchJob(1, 1000, 2)])
+    with pytest.raises(M.McflError):
+        M.Batch(ctx, [sim], [M.BatchJob(0, 1000, 1, src_kind=M.SOURCE_EMISSION, Fx=np.ones(sim.nvox))])
+    with pytest.raises(M.McflError):
+        M.Batch(ctx, [sim], [M.BatchJob(0, 1000, 1, source=(0.0, 0.0, 50.0))])
 
 
 def test_zero_and_one_photon(ctx):
