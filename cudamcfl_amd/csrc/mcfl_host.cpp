@@ -367,44 +367,66 @@ extern "C" const char* mcfl_default_primes_path(void) {
 
 extern "C" uint64_t mcfl_stream_key(uint64_t seed, uint64_t stream) { return ::mcfl_key(seed, stream); }
 
-// init_RNG, CUDAMCFLrng.cu:31-90.
+namespace {
+
+// One line of the multipliers file: the multiplier in its first column.
+bool read_multiplier(FILE* fp, uint32_t* a) {
+  unsigned int m, p1, p2;
+  if (fscanf(fp, "%u %u %u", &m, &p1, &p2) != 3) return false;
+  *a = m;
+  return true;
+}
+
+// Lane state drawn from the seeding generator g (multiplier `seeder`) for a
+// lane of multiplier a: one step gives the carry as floor(u / 2^32 * a),
+// evaluated in double precision as CUDAMCFLrng.cu:78 does, the next step's
+// low word is the state's low word (:79-81).  Draws repeat until the state
+// is valid for a (CUDAMCFLrng.cu:72).
+uint64_t seed_lane(uint64_t* g, uint32_t seeder, uint32_t a) {
+  uint64_t x;
+  do {
+    const uint32_t u = mcfl_mwc_next(g, seeder);
+    const uint32_t carry = (uint32_t)floor((double)u / 4294967296.0 * (double)a);
+    const uint32_t lo = mcfl_mwc_next(g, seeder);
+    x = ((uint64_t)carry << 32) | lo;
+  } while (!mcfl_mwc_state_ok(x, a));
+  return x;
+}
+
+}  // namespace
+
+// init_RNG, CUDAMCFLrng.cu:31-90: the file's first multiplier drives a
+// seeding generator started at xinit (which must itself be a valid state for
+// it, :56-65); lane i takes the multiplier on line i+2 and a state drawn from
+// the seeding generator.
 extern "C" int mcfl_init_rng(uint64_t* x, uint32_t* a, uint32_t n_rng, const char* file, uint64_t xinit) {
   if (!file || !file[0]) file = mcfl_default_primes_path();
   FILE* fp = fopen(file, "r");
   if (!fp) {
-    printf("Could not find the file of safeprimes (%s)! Terminating!\n", file);
+    fprintf(stderr, "mcfl: cannot open the multipliers file %s\n", file);
     return 1;
   }
-  unsigned int begin = 0u, fora = 0u, t1, t2;
-  if (fscanf(fp, "%u %u %u", &begin, &t1, &t2) != 3) {
-    fclose(fp);
-    return 1;
-  }
-  if ((xinit == 0ull) | (((unsigned int)(xinit >> 32)) >= (begin - 1)) | (((unsigned int)xinit) >= 0xfffffffful)) {
-    printf("%llu not a valid seed! Terminating!\n", (unsigned long long)xinit);
-    fclose(fp);
-    return 1;
-  }
-  for (uint32_t i = 0; i < n_rng; i++) {
+  uint32_t seeder = 0;
+  int rc = 0;
+  if (!read_multiplier(fp, &seeder)) {
+    rc = 1;
+  } else if (!mcfl_mwc_state_ok(xinit, seeder)) {
+    fprintf(stderr, "mcfl: seed %llu is not a valid state for the seeding multiplier\n", (unsigned long long)xinit);
+    rc = 1;
+  } else {
+    uint64_t g = xinit;
     // The reference keeps the previous multiplier when the file runs out; a
     // short file is an error here.
-    if (fscanf(fp, "%u %u %u", &fora, &t1, &t2) != 3) {
-      fclose(fp);
-      return 1;
+    for (uint32_t i = 0; i < n_rng && rc == 0; i++) {
+      if (!read_multiplier(fp, &a[i])) {
+        rc = 1;
+        break;
+      }
+      x[i] = seed_lane(&g, seeder, a[i]);
     }
-    a[i] = fora;
-    uint64_t xi = 0;
-    while ((xi == 0) | (((unsigned int)(xi >> 32)) >= (fora - 1)) | (((unsigned int)xi) >= 0xfffffffful)) {
-      xinit = (xinit & 0xffffffffull) * begin + (xinit >> 32);
-      xi = (unsigned int)floor((((double)((unsigned int)xinit)) / (double)0x100000000) * fora);
-      xi <<= 32;
-      xinit = (xinit & 0xffffffffull) * begin + (xinit >> 32);
-      xi += (unsigned int)xinit;
-    }
-    x[i] = xi;
   }
   fclose(fp);
-  return 0;
+  return rc;
 }
 
 extern "C" const char* mcfl_status_string(mcfl_status s) {

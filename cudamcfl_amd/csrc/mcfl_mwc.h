@@ -50,6 +50,12 @@ MCFL_HD uint64_t mcfl_key(uint64_t seed, uint64_t stream) {
   return mcfl_mix64(mcfl_mix64(seed ^ 0xD1B54A32D192ED03ull) + (stream + 1ull) * MCFL_GOLDEN64);
 }
 
+/* A state x is usable with multiplier a when it is not 0, its carry is below
+   a-1 and its low word below 2^32-1 (CUDAMCFLrng.cu:56-60,72). */
+MCFL_HD int mcfl_mwc_state_ok(uint64_t x, uint32_t a) {
+  return x != 0ull && (uint32_t)(x >> 32) < a - 1u && (uint32_t)x != 0xFFFFFFFFu;
+}
+
 /* Valid MWC state for photon k of the job with key `key`, multiplier a. */
 MCFL_HD uint64_t mcfl_photon_state(uint64_t key, uint64_t k, uint32_t a) {
   uint64_t h = key + (k + 1ull) * MCFL_GOLDEN64;
@@ -57,7 +63,8 @@ MCFL_HD uint64_t mcfl_photon_state(uint64_t key, uint64_t k, uint32_t a) {
     uint64_t z = mcfl_mix64(h);
     uint32_t lo = (uint32_t)z;
     uint32_t c = (uint32_t)(((uint64_t)(uint32_t)(z >> 32) * (uint64_t)a) >> 32);
-    if (!((lo == 0u && c == 0u) || c >= a - 1u || lo == 0xFFFFFFFFu)) return ((uint64_t)c << 32) | lo;
+    const uint64_t x = ((uint64_t)c << 32) | lo;
+    if (mcfl_mwc_state_ok(x, a)) return x;
     h += MCFL_GOLDEN64;
   }
 }

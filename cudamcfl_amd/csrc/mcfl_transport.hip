@@ -233,7 +233,15 @@ __device__ uint32_t glass_march(const MediumDev& m, Ph& p, uint32_t old_lab, flo
   float moved = 0.0f;
   int64_t index = 1;
   uint32_t present = old_lab;
-  while (present == old_lab && moved < max_s) {
+  /* A march inside a region whose label is an exit (0 or n_bulks+1, e.g. a
+     photon launched there) never sees its label change: at the surface its
+     step shrinks to nothing and the reference (CUDAMCFLtransport.cu:717)
+     loops for ever.  Every other march advances 1/(2 gs) per pass and ends
+     within about max_s * 2 gs passes (the float sum of the steps drifts by
+     far less than that), so twice that bound only stops the endless ones. */
+  const uint32_t passes = (uint32_t)(max_s * 4.0f * m.gs) + 64u;
+  uint32_t pass = 0;
+  while (present == old_lab && moved < max_s && pass++ < passes) {
     p.x = mcfl_fma(p.dx, step, p.x);
     p.y = mcfl_fma(p.dy, step, p.y);
     p.z = mcfl_fma(p.dz, step, p.z);

@@ -277,7 +277,13 @@ static uint32_t orc_move_to_boundary(const ctx_t* c, orc_photon* p, uint32_t old
   float total_move = 0.0f;
   int64_t index = 1;
   uint32_t present = old_bulk;
-  while (present == old_bulk && total_move < max_s) { /* :717-756 */
+  /* Bounded: a photon inside an exit-labelled region (0 or n_bulks+1)
+   * marches at a surface with a step of zero and the reference never leaves
+   * the loop; any other march ends within about max_s*2gs passes, so twice
+   * that only stops the endless ones. */
+  const uint32_t passes = (uint32_t)(max_s * 4.0f * s->gs) + 64u;
+  uint32_t pass = 0;
+  while (present == old_bulk && total_move < max_s && pass++ < passes) { /* :717-756 */
     p->x = mcfl_fma(p->dx, search_step, p->x);
     p->y = mcfl_fma(p->dy, search_step, p->y);
     p->z = mcfl_fma(p->dz, search_step, p->z);

@@ -822,6 +822,51 @@ def test_glass_march_lands_on_the_backoff_sphere():
         assert abs(tof - (k * step_t - max(0, int(back / c_cmfs)))) <= 1, (tof, k)
 
 
+def test_glass_march_bound_ends_only_endless_marches():
+    """The march's pass bound (4 max_s gs + 64 passes).  A photon inside an
+    exit-labelled region (label 0 above, n_bulks+1 below) that marches to the
+    slab's surface never sees its label change, and the reference's loop
+    (CUDAMCFLtransport.cu:717-756) steps by -z/dz = 0 at the surface for
+    ever: the bounded march returns the exit label with the photon on the
+    surface.  A glass march that leaves the grid sideways in a glass label 1
+    (the label of the space beside the grid, :741-744) runs until the float
+    sum of its 0.05 cm steps reaches max_s = 100 cm, 2000 passes, as in the
+    reference: the bound must not cut it short (a bound of max_s gs passes
+    fails the time of flight)."""
+    f32 = np.float32
+    glass = [FLT_MAX, 0.0, 1.0, 1.5, 0, 0, 0, 1]
+    lab = np.full((10, 8, 8), 2, np.uint8)
+    lab[:4] = 0  # exit label 0 in the top 4 voxel layers, 3 (= n_bulks+1) in the bottom 3
+    lab[7:] = 3
+    b = [[FLT_MAX, 0, 1, 1.0, 0, 0, 0, 0], glass, [f32(1) / f32(100.1), 0.1, 0.9, 1.4, 0, 0.1, 0.1, 1],
+         [FLT_MAX, 0, 1, 1.0, 0, 0, 0, 0]]
+    m = O.Medium(lab, b, 1.0, 10, halfx=0.4, halfy=0.4)
+    sim = m.sim()
+    for p0, d0, old, z_end in (((0.01, 0.02, 0.25), (0.1, -0.2, -0.97), 0, 0.0),
+                               ((-0.03, 0.01, 0.85), (0.2, 0.1, 0.97), 3, 1.0)):
+        d = np.array(d0, np.float32)
+        d = (d / np.float32(np.sqrt(np.sum(d.astype(np.float64) ** 2)))).astype(np.float32)
+        lab_out, out, _ = O.march(sim, np.array(p0, np.float32), d, old)
+        assert lab_out == old
+        assert abs(float(out[2]) - z_end) < 1e-6
+        t = (z_end - p0[2]) / float(d[2])  # the ray's crossing of the surface
+        assert np.max(np.abs(out[:2] - (np.array(p0[:2]) + t * d[:2].astype(np.float64)))) < 1e-5
+    # the long sideways march in glass label 1
+    lab1 = np.ones((10, 8, 8), np.uint8)
+    m1 = O.Medium(lab1, [b[0], glass, b[3]], 1.0, 10, halfx=0.4, halfy=0.4)
+    sim1 = m1.sim()
+    d = np.array([0.6, 0.8, 1e-4], np.float32)
+    lab_out, out, tof = O.march(sim1, np.array([0.0, 0.0, 0.5], np.float32), d, 1)
+    h = f32(1) / (f32(2) * f32(10))
+    total, k = f32(0), 0
+    while total < f32(100):
+        total = f32(total + h)
+        k += 1
+    assert lab_out == 1 and 1990 <= k <= 2010
+    assert tof == k * int(float(h) / 2.99792e-5)
+    assert abs(float(out[0]) - k * float(h) * 0.6) < 1e-3 * k * float(h)
+
+
 def test_emission_loop_images_by_hand(a_tab):
     """The reference's emission loop (CUDAMCFLmain.cu:471-594) on a 3x1x1
     medium whose photons all leave on their first step (mean free path
