@@ -33,7 +33,9 @@
    default), 1024 (one workgroup per CU, 4 waves/SIMD, one 160 KB tile). */
 #define MCFL_WG_ALT1 512
 #define MCFL_WG_ALT2 1024
+#ifndef MCFL_TALLY_COPIES
 #define MCFL_TALLY_COPIES 8      /* per-step volume tallies: one copy per XCD, summed after the launch */
+#endif
 
 enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2, MCFL_WORK_VLANES = 3 };
 enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2, MCFL_SRC_EMISSION = 3 };
@@ -95,6 +97,18 @@ struct DetDev {
 
 /* Tally flags */
 enum { MCFL_F_FHD = 1, MCFL_F_AXYZ = 2, MCFL_F_SQ = 4, MCFL_F_TEMP = 8, MCFL_F_SCALED = 16 };
+
+/* What the reflectance / transmittance image tally of an exit reads, one
+   per job, copied from the job table into LDS when a workgroup starts: an
+   exit then reads its detector from LDS instead of a chain of global loads. */
+struct ExitDev {
+  float x0, y0, size_x, size_y, dx, dy;
+  uint32_t nx, npix, flags, pad;
+  unsigned long long* Rd;
+  unsigned long long* Tt;
+  unsigned long long* Rd2;
+  unsigned long long* Tt2;
+};
 
 struct JobDev {
   uint64_t begin;  /* first flattened photon index */
@@ -196,12 +210,14 @@ enum : uint64_t {
   MCFL_NA_BYTES = MCFL_NA_OFF_VOL + MCFL_NA_VOL_BYTES /* 73296896 bytes, 69.9 MiB */
 };
 
-/* Dynamic LDS of a launch: per-job totals, fhd tile, material table, layer bounds. */
 /* Threads per workgroup of a launch. */
 static inline uint32_t mcfl_wg_of(const LaunchArgs* A) { return A->wg ? A->wg : (uint32_t)MCFL_WG; }
 
+/* Dynamic LDS of a launch: per-job totals, fhd tile, material table, layer
+   bounds, per-job exit constants. */
 static inline uint32_t mcfl_lds_bytes(uint32_t nslots, uint32_t tile_n, uint32_t n_mats, uint32_t n_zb) {
-  return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * (uint32_t)sizeof(MatDev) + n_zb * 8u;
+  return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * (uint32_t)sizeof(MatDev) + n_zb * 8u +
+         nslots * (uint32_t)sizeof(ExitDev);
 }
 
 #endif

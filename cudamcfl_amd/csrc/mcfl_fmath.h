@@ -118,9 +118,8 @@ MCFL_HD float mcfl_renorm(float x) {
   return mcfl_fabs(x - 1.0f) < 0x1p-12f ? mcfl_fma(x, -0.5f, 1.5f) : mcfl_rsqrt(x);
 }
 
-/* sin(2*pi*t) and cos(2*pi*t). */
-MCFL_HD void mcfl_sincos_turn(float t, float* sn, float* cs) {
-  float y = 4.0f * t;
+/* sin((pi/2) y) and cos((pi/2) y) for y = 4t in [0, 4]. */
+MCFL_HD void mcfl_sincos_quarter(float y, float* sn, float* cs) {
   float j = mcfl_rint(y);
   float r = y - j; /* exact, |r| <= 1/2; angle = (pi/2)(j + r) */
   int q = (int)j & 3;
@@ -142,14 +141,34 @@ MCFL_HD void mcfl_sincos_turn(float t, float* sn, float* cs) {
   *cs = c;
 }
 
+/* sin(2*pi*t) and cos(2*pi*t). */
+MCFL_HD void mcfl_sincos_turn(float t, float* sn, float* cs) { mcfl_sincos_quarter(4.0f * t, sn, cs); }
+
 /* ---- conversions with the saturating semantics of the GPU instructions ---- */
 
-/* uint32 -> float, rounding toward zero: keep the 24 leading bits, then the
-   conversion is exact.  0xFFFFFF00 >> clz(u) has ones from the leading bit
-   down 24 places (bits above the leading one are zero in u anyway); u | 1
-   keeps clz defined at u = 0. */
+/* uint32 -> float, rounding toward zero.  On the host: keep the 24 leading
+   bits, then the conversion is exact (0xFFFFFF00 >> clz(u) has ones from the
+   leading bit down 24 places, bits above the leading one are zero in u
+   anyway; u | 1 keeps clz defined at u = 0).  On the GPU: the conversion
+   instruction with the FP32 rounding field of the MODE register (bits 1:0)
+   set to toward-zero around it, one vector instruction and two scalar ones
+   instead of five vector ones.  The three are one asm statement: the
+   compiler does not order the conversion after a MODE write made with the
+   builtin, and has moved it out of such a window.  Both forms agree for all
+   2^32 inputs (checked on the GPU). */
 MCFL_HD float mcfl_u2f_rz(uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  float f;
+  asm volatile(
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 3\n\t"
+      "v_cvt_f32_u32 %0, %1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 0"
+      : "=v"(f)
+      : "v"(u));
+  return f;
+#else
   return (float)(u & (0xFFFFFF00u >> __builtin_clz(u | 1u)));
+#endif
 }
 
 /* float -> uint32 toward zero; NaN and negatives give 0, >= 2^32 saturates. */

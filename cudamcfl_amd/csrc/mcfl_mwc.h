@@ -31,8 +31,17 @@ MCFL_HD uint32_t mcfl_mwc_next(uint64_t* x, uint32_t a) {
 
 /* [0,1) */
 MCFL_HD float mcfl_rand_co(uint64_t* x, uint32_t a) { return mcfl_u2f_rz(mcfl_mwc_next(x, a)) * 0x1p-32f; }
-/* (0,1] */
-MCFL_HD float mcfl_rand_oc(uint64_t* x, uint32_t a) { return 1.0f - mcfl_rand_co(x, a); }
+/* (0,1]: 1 - that, one FMA (the product with 2^-32 is exact, so this is the
+   same rounding as 1.0f - mcfl_rand_co) */
+MCFL_HD float mcfl_rand_oc(uint64_t* x, uint32_t a) {
+  return mcfl_fma(mcfl_u2f_rz(mcfl_mwc_next(x, a)), -0x1p-32f, 1.0f);
+}
+
+/* sin and cos of 2 pi times a [0,1) draw: the quarter-turn argument
+   4 * draw = u * 2^-30 straight from the converted integer (exact) */
+MCFL_HD void mcfl_rand_sincos(uint64_t* x, uint32_t a, float* sn, float* cs) {
+  mcfl_sincos_quarter(mcfl_u2f_rz(mcfl_mwc_next(x, a)) * 0x1p-30f, sn, cs);
+}
 
 MCFL_HD uint64_t mcfl_mix64(uint64_t z) {
   z ^= z >> 30;

@@ -87,6 +87,10 @@ __device__ __forceinline__ bool in_sphere(const MediumDev& m, float x, float y, 
 }
 
 typedef __attribute__((address_space(1))) unsigned long long gu64;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+
+/* p[i] through a global (not generic) pointer: a global load instead of a flat one */
+__device__ __forceinline__ uint32_t gload_u32(const uint32_t* p, uint32_t i) { return ((gcu32*)p)[i]; }
 
 __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v) {
   __hip_atomic_fetch_add((gu64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -155,7 +159,7 @@ __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint3
   if (mt.twog == 0.0f) cost = 2.0f * mcfl_rand_co(&x, a) - 1.0f;
   const float s2 = mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)); /* sin^2 theta: 0 or >= 2^-23 */
   float sps, cps;
-  mcfl_sincos_turn(mcfl_rand_co(&x, a), &sps, &cps);
+  mcfl_rand_sincos(&x, a, &sps, &cps);
   const float cosp = sps, sinp = cps; /* the reference's __sincosf argument order */
   const float dx = p.dx, dy = p.dy, dz = p.dz;
   const float q = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)); /* temp^2 = 1 - dz^2: 0 or >= 6e-8 */
@@ -295,23 +299,25 @@ __device__ uint32_t glass_march(const MediumDev& m, Ph& p, uint32_t old_lab, flo
   return present;
 }
 
-/* Reflectance / transmittance exit tally, CUDAMCFLtransport.cu:369-421. */
+/* Reflectance / transmittance exit tally, CUDAMCFLtransport.cu:369-421: the
+   image from the job's exit constants in LDS (E), the temporal detectors
+   from its job table entry (J). */
 template <bool TEMPORAL>
-__device__ void exit_tally(const JobDev& J, const Ph& p, bool top, bool scaled, double escale,
+__device__ void exit_tally(const JobDev& J, const ExitDev& E, const Ph& p, bool top, bool scaled, double escale,
                            unsigned long long* st) {
-  const DetDev& d = J.det;
   lds_add_u64(&st[top ? MCFL_DT_R : MCFL_DT_T], p.w);
-  if (mcfl_fabs(p.x - d.x0) < d.size_x && mcfl_fabs(p.y - d.y0) < d.size_y) {
-    const uint32_t iy = mcfl_f2u_rz((p.y - d.y0 + d.size_y) / d.dy);
-    const uint32_t ix = mcfl_f2u_rz((p.x - d.x0 + d.size_x) / d.dx);
-    const uint64_t idx = (uint64_t)iy * (uint64_t)d.nx + ix;
-    if (idx < d.npix) {
+  if (mcfl_fabs(p.x - E.x0) < E.size_x && mcfl_fabs(p.y - E.y0) < E.size_y) {
+    const uint32_t iy = mcfl_f2u_rz((p.y - E.y0 + E.size_y) / E.dy);
+    const uint32_t ix = mcfl_f2u_rz((p.x - E.x0 + E.size_x) / E.dx);
+    const uint64_t idx = (uint64_t)iy * (uint64_t)E.nx + ix;
+    if (idx < E.npix) {
       unsigned long long inc = scaled ? (unsigned long long)((double)p.w * escale + 0.5) : (unsigned long long)p.w;
-      atomic_add_u64((top ? J.Rd : J.Tt) + idx, inc);
-      if (J.flags & MCFL_F_SQ)
-        atomic_add_u64((top ? J.Rd2 : J.Tt2) + idx, ((unsigned long long)p.w * (unsigned long long)p.w) >> 32);
+      atomic_add_u64((top ? E.Rd : E.Tt) + idx, inc);
+      if (E.flags & MCFL_F_SQ)
+        atomic_add_u64((top ? E.Rd2 : E.Tt2) + idx, ((unsigned long long)p.w * (unsigned long long)p.w) >> 32);
     }
   }
+  const DetDev& d = J.det;
   if (TEMPORAL && (J.flags & MCFL_F_TEMP) && d.t_rort == (top ? 0u : 1u) && p.tof < d.t_max) {
     for (uint32_t xp = 0; xp < d.t_ndx; ++xp)
       for (uint32_t yp = 0; yp < d.t_ndy; ++yp) {
@@ -350,7 +356,7 @@ constexpr int waves_per_eu_of(int wg) { return wg > 768 ? 4 : MCFL_WAVES_PER_EU;
 
 template <int WORK, bool TEMPORAL, bool PERJOB, bool LAYERED, int WGS>
 __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_eu_of(WGS)))) void mcfl_transport_kernel(LaunchArgs A) {
-  /* dynamic LDS: [per-job totals | fhd tile | materials | layer bounds] */
+  /* dynamic LDS: [per-job totals | fhd tile | materials | layer bounds | per-job exit constants] */
   extern __shared__ __align__(16) unsigned long long lds[];
 
   const uint32_t tid = threadIdx.x;
@@ -359,7 +365,27 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   unsigned long long* stile = lds + nslots * MCFL_NTOT;
   MatDev* smat = reinterpret_cast<MatDev*>(stile + A.tile_n);
   float* szb = reinterpret_cast<float*>(smat + A.n_mats);
+  ExitDev* sexit = reinterpret_cast<ExitDev*>(szb + (LAYERED ? 2u * (A.med.n_layers + 2u) : 0u));
   for (uint32_t i = tid; i < A.n_mats; i += WGS) smat[i] = A.mats[i];
+  for (uint32_t i = tid; i < nslots; i += WGS) {
+    const JobDev& J = A.jobs[i];
+    ExitDev e;
+    e.x0 = J.det.x0;
+    e.y0 = J.det.y0;
+    e.size_x = J.det.size_x;
+    e.size_y = J.det.size_y;
+    e.dx = J.det.dx;
+    e.dy = J.det.dy;
+    e.nx = (uint32_t)J.det.nx;
+    e.npix = J.det.npix;
+    e.flags = J.flags;
+    e.pad = 0u;
+    e.Rd = J.Rd;
+    e.Tt = J.Tt;
+    e.Rd2 = J.Rd2;
+    e.Tt2 = J.Tt2;
+    sexit[i] = e;
+  }
   if (LAYERED)
     for (uint32_t i = tid; i < 2u * (A.med.n_layers + 2u); i += WGS) szb[i] = A.med.zb[i];
   for (uint32_t i = tid; i < nslots * MCFL_NTOT + A.tile_n; i += WGS) lds[i] = 0ull;
@@ -407,6 +433,9 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
      when it leaves the voxel or dies.  Integer sums are unchanged. */
   uint32_t fv = 0xFFFFFFFFu, av = 0xFFFFFFFFu, ft = 0xFFFFFFFFu;
   unsigned long long facc = 0ull, aacc = 0ull;
+  /* the lane's last label lookup (voxel, label); cleared for each photon, whose
+     medium may differ from the last one's */
+  uint32_t lv = 0xFFFFFFFFu, ll = 0u;
   uint32_t iters = 0;
   bool alive = false, retired = false;
   uint32_t lane_item = 0;
@@ -485,8 +514,11 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
         accA = 0;
         iters = 0;
         if (WORK == MCFL_WORK_JOBS) {
+          /* the job table entry through the wave-uniform cur: scalar loads,
+             which do not wait behind the wave's outstanding vector memory
+             operations (tally atomics) */
           job = cur;
-          const JobDev& J = A.jobs[job];
+          const JobDev& J = A.jobs[cur];
           const uint64_t k = g + J.k0;
           a = A.a_tab[k & A.a_mask];
           x = mcfl_photon_state(J.key, k, a);
@@ -500,7 +532,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
                isotropic photon from its corner as in CUDAMCFLmain.cu:481-487 */
             const uint32_t col = (uint32_t)(((uint64_t)mcfl_mwc_next(&x, a) * (uint64_t)J.al_n) >> 32);
             const uint32_t u = mcfl_mwc_next(&x, a);
-            const uint32_t v = J.al_vox[u < J.al_prob[col] ? col : J.al_alias[col]];
+            const uint32_t v = gload_u32(J.al_vox, u < gload_u32(J.al_prob, col) ? col : gload_u32(J.al_alias, col));
             const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
             const uint32_t iz = v / nxy, rem = v - iz * nxy, iy = rem / (uint32_t)m.nx, ix = rem - iy * (uint32_t)m.nx;
             launch(m, MCFL_SRC_ISOTROPIC, ((float)ix / m.gs) - m.halfx, ((float)iy / m.gs) - m.halfy,
@@ -543,6 +575,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           uint64_t xl = x; /* LaunchPhoton_Global does not store the advanced state */
           launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, xl, a, LAYERED, J.start_lab);
         }
+        lv = 0xFFFFFFFFu;
         unsigned long long* st = stot + job * MCFL_NTOT;
         lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
         lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
@@ -566,6 +599,28 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     bool in;
     Cell cl;
     uint32_t vi;
+    /* hitting density: the pre-absorption weight at the step's end point,
+       summed per lane while the photon stays in one voxel (CUDAMCFLtransport.cu:358-366) */
+    auto fhd_visit = [&]() {
+      if ((flags & MCFL_F_FHD) && in) {
+        if (vi != fv) {
+          if (facc != 0ull) {
+            if (ft != 0xFFFFFFFFu)
+              lds_add_u64(stile + ft, facc);
+            else
+              atomic_add_u64(fhd + fv, facc);
+          }
+          fv = vi;
+          facc = 0ull;
+          ft = 0xFFFFFFFFu;
+          if (job == tile_job) { /* inside the LDS tile of the workgroup's home job? */
+            const uint32_t lx = cl.ix - tx0, ly = cl.iy - ty0, lz = cl.iz;
+            if (lx < A.tnx && ly < A.tny && lz < A.tnz) ft = lx + __umul24(A.tnx, ly + __umul24(A.tny, lz));
+          }
+        }
+        facc += p.w;
+      }
+    };
     if constexpr (LAYERED) {
       const uint32_t L = p.lab;
       const float zlo = szb[2u * L], zhi = szb[2u * L + 1u];
@@ -623,7 +678,25 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       in = in_grid(m, p.x, p.y, p.z);
       cl = cell_in(m, p.x, p.y, p.z);
       vi = in ? cell_index(m, cl) : 0u;
-      if (nb == p.lab && mc.mutr != kFltMax) nb = in ? (uint32_t)m.labels[vi] : 1u; /* :319-333 */
+      /* the end point's label (:319-333): gathered unless it is the voxel of
+         the lane's last lookup; the hitting-density update, which does not
+         depend on it, runs while the gather is in flight */
+      const bool look = nb == p.lab && mc.mutr != kFltMax;
+      const bool gather = look && in && vi != lv;
+      uint32_t got = 0u;
+      if (gather) got = (uint32_t)m.labels[vi];
+      fhd_visit();
+      if (look) {
+        if (!in) {
+          nb = 1u;
+        } else if (gather) {
+          nb = got;
+          lv = vi;
+          ll = got;
+        } else {
+          nb = ll;
+        }
+      }
       mm = mc; /* material after a possible interface crossing */
       if (nb != p.lab) {
         reflect<!(TEMPORAL && PERJOB)>(mats, p, nb, m.last, x, a);
@@ -631,27 +704,10 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       }
     }
 
-    if ((flags & MCFL_F_FHD) && in) {
-      if (vi != fv) {
-        if (facc != 0ull) {
-          if (ft != 0xFFFFFFFFu)
-            lds_add_u64(stile + ft, facc);
-          else
-            atomic_add_u64(fhd + fv, facc);
-        }
-        fv = vi;
-        facc = 0ull;
-        ft = 0xFFFFFFFFu;
-        if (job == tile_job) { /* inside the LDS tile of the workgroup's home job? */
-          const uint32_t lx = cl.ix - tx0, ly = cl.iy - ty0, lz = cl.iz;
-          if (lx < A.tnx && ly < A.tny && lz < A.tnz) ft = lx + __umul24(A.tnx, ly + __umul24(A.tny, lz));
-        }
-      }
-      facc += p.w;
-    }
+    if constexpr (LAYERED) fhd_visit();
 
     if (p.lab == 0u || p.lab == m.last) {
-      exit_tally<TEMPORAL>(A.jobs[job], p, p.lab == 0u, (flags & MCFL_F_SCALED) != 0u, escale, st);
+      exit_tally<TEMPORAL>(A.jobs[job], sexit[job], p, p.lab == 0u, (flags & MCFL_F_SCALED) != 0u, escale, st);
       p.w = 0u;
       s = 0.0f;
     }

@@ -178,7 +178,7 @@ static void orc_spin(const mat_t* m, orc_photon* p, uint64_t* x, uint32_t a) {
   if (m->g == 0.0f) cost = 2.0f * mcfl_rand_co(x, a) - 1.0f;
   float sin2t = mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)); /* sint^2 (:600) */
   float s_psi, c_psi;
-  mcfl_sincos_turn(mcfl_rand_co(x, a), &s_psi, &c_psi);
+  mcfl_rand_sincos(x, a, &s_psi, &c_psi);
   float cosp = s_psi; /* :603 assigns the sine to cosp */
   float sinp = c_psi;
   float dx = p->dx, dy = p->dy, dz = p->dz;
