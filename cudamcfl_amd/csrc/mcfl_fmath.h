@@ -171,9 +171,18 @@ MCFL_HD float mcfl_u2f_rz(uint32_t u) {
 #endif
 }
 
-/* float -> uint32 toward zero; NaN and negatives give 0, >= 2^32 saturates. */
+/* float -> uint32 toward zero; NaN and negatives give 0, >= 2^32 saturates.
+   On the GPU that is what the conversion instruction does by itself (equal
+   to the host form for all 2^32 inputs, checked on the GPU); written as asm
+   because the C cast of an out-of-range value is undefined. */
 MCFL_HD uint32_t mcfl_f2u_rz(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t r;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(f));
+  return r;
+#else
   return (f >= 4294967296.0f) ? 0xFFFFFFFFu : ((f > 0.0f) ? (uint32_t)f : 0u);
+#endif
 }
 MCFL_HD uint32_t mcfl_f2u_ru(float f) { return mcfl_f2u_rz(mcfl_ceil(f)); }
 MCFL_HD uint32_t mcfl_f2u_rn(float f) { return mcfl_f2u_rz(mcfl_rint(f)); }

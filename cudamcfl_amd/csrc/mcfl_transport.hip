@@ -64,7 +64,7 @@ __device__ __forceinline__ uint32_t vox_of(const MediumDev& m, float x, float y,
 /* Cell of an in-grid point (|x| < halfx, |y| < halfy, z < esp), the same
    integers as vox_of: x + halfx and y + halfy are >= 0, so the truncating
    conversion needs no saturation; z can be a rounding below 0 on the top
-   surface, hence the max. */
+   surface, which the saturating conversion takes to 0. */
 struct Cell {
   uint32_t ix, iy, iz;
 };
@@ -72,7 +72,7 @@ __device__ __forceinline__ Cell cell_in(const MediumDev& m, float x, float y, fl
   Cell c;
   c.ix = min((uint32_t)((x + m.halfx) * m.gs), (uint32_t)m.nx - 1u);
   c.iy = min((uint32_t)((y + m.halfy) * m.gs), (uint32_t)m.ny - 1u);
-  c.iz = min((uint32_t)mcfl_fmax(z * m.gs, 0.0f), (uint32_t)m.nz - 1u);
+  c.iz = min(mcfl_f2u_rz(z * m.gs), (uint32_t)m.nz - 1u);
   return c;
 }
 /* nx, ny < 2^24 and nx*ny*nz < 2^32 (host validation) keep the products exact */
