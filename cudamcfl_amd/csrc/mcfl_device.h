@@ -95,6 +95,10 @@ struct DetDev {
   const float* t_posy;
 };
 
+/* Self-checks and evaluations of the per-step arithmetic (test support) */
+enum { MCFL_SC_U2F_RZ = 0, MCFL_SC_F2U_RZ = 1, MCFL_SC_DIV = 2, MCFL_SC_PIXEL = 3 };
+enum { MCFL_ME_LOG = 0, MCFL_ME_RSQRT = 1, MCFL_ME_RENORM = 2, MCFL_ME_SINCOS = 3, MCFL_ME_DRAW_OC = 4 };
+
 /* Tally flags */
 enum { MCFL_F_FHD = 1, MCFL_F_AXYZ = 2, MCFL_F_SQ = 4, MCFL_F_TEMP = 8, MCFL_F_SCALED = 16 };
 
@@ -108,6 +112,7 @@ struct ExitDev {
   unsigned long long* Tt;
   unsigned long long* Rd2;
   unsigned long long* Tt2;
+  double rdx, rdy; /* 1/dx, 1/dy rounded to double */
 };
 
 struct JobDev {

@@ -14,7 +14,8 @@
  * tallies of a fixed photon set agree exactly between CPU and GPU.
  *
  *  - logarithm: exponent/mantissa split with the mantissa folded into
- *    [sqrt(1/2), sqrt(2)), log1p(f) = f + f^2 R(f) with R a degree-8
+ *    (sqrt(1/2), sqrt(2)] by one integer subtraction of the bits of
+ *    sqrt(1/2), log1p(f) = f + f^2 R(f) with R a degree-8
  *    polynomial fitted by least squares on Chebyshev nodes (near-minimax,
  *    Remez-style approximation), Cody-Waite two-constant ln2 split.
  *    Max relative error about 2 ulp on (0, 1].
@@ -70,17 +71,21 @@ MCFL_HD float mcfl_ceil(float x) { return __builtin_ceilf(x); }
 MCFL_HD float mcfl_fabs(float x) { return __builtin_fabsf(x); }
 MCFL_HD float mcfl_copysign(float m, float s) { return __builtin_copysignf(m, s); }
 MCFL_HD float mcfl_fmax(float a, float b) { return a > b ? a : b; }
+/* max(x, 0) as one maximum instruction (IEEE maxNum: a NaN x gives 0).  For
+   x that is never -0, where maxNum may return either zero: the call sites
+   take x = 1 - c*c from one FMA, whose exact zero rounds to +0. */
+MCFL_HD float mcfl_clamp0(float x) { return __builtin_fmaxf(x, 0.0f); }
 
-/* Natural logarithm for positive normal floats. */
+/* Natural logarithm for positive normal floats.  u = 2^e m with m in
+   [0x3F3504F4, 0x3FB504F3] (as bits, about [sqrt(1/2), sqrt(2)]): the bits of
+   u less those of the lowest m keep m's mantissa field in their low 23 bits
+   and e in the bits above, an arithmetic shift's worth of exponent.  (The
+   same e and m as splitting u at 1 and halving a mantissa above
+   0x3FB504F3; both forms agree for every normal positive float.) */
 MCFL_HD float mcfl_log(float u) {
-  uint32_t b = mcfl_fbits(u);
-  int e = (int)(b >> 23) - 127;
-  uint32_t mb = (b & 0x007FFFFFu) | 0x3F800000u; /* mantissa in [1,2) */
-  if (mb > 0x3FB504F3u) {                        /* above sqrt(2): halve */
-    mb -= 0x00800000u;
-    e += 1;
-  }
-  float f = mcfl_bitsf(mb) - 1.0f; /* exact (Sterbenz) */
+  const uint32_t t = mcfl_fbits(u) - 0x3F3504F4u;
+  const int e = (int32_t)t >> 23;
+  const float f = mcfl_bitsf((t & 0x007FFFFFu) + 0x3F3504F4u) - 1.0f; /* exact (Sterbenz) */
   float r = -0.07473210990428925f;
   r = mcfl_fma(r, f, 0.128195121884346f);
   r = mcfl_fma(r, f, -0.1326199471950531f);

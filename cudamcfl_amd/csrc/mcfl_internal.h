@@ -20,6 +20,13 @@ extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stre
 // Device address of the current device's allocation-free scratch (MCFL_NA_BYTES).
 extern "C" int mcfl_scratch_address(int device, void** out, uint64_t* bytes);
 extern "C" int mcfl_reduce_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, hipStream_t stream);
+// Test support (no counterpart in the reference): mcfl_selfcheck counts, over
+// inputs [lo, hi) (32-bit patterns), where a form of the per-step arithmetic
+// the transport kernel uses differs on the GPU from an independent form
+// (which: MCFL_SC_*; p the fixed operand's bits); mcfl_math_eval evaluates the
+// shared FP32 functions (fn: MCFL_ME_*) on the GPU for comparison with the host.
+extern "C" int mcfl_selfcheck(int which, uint32_t p, uint64_t lo, uint64_t hi, uint64_t* mismatches, uint64_t* first);
+extern "C" int mcfl_math_eval(int fn, const uint32_t* in, uint32_t* out, uint64_t n);
 // acc[i] += the sum of the ncopies copies of element i, both saturating at
 // 2^64-1; the copies are zeroed (ready for the next launch of a chunked run).
 extern "C" int mcfl_accumulate_copies(unsigned long long* p, uint64_t n, uint32_t ncopies, unsigned long long* acc,

@@ -151,18 +151,33 @@ __device__ __forceinline__ void launch_voxel(const MediumDev& m, uint32_t v, Ph&
          0u, p, x, a, false, 0u);
 }
 
+/* a / b, the correctly rounded quotient, for a and b whose quotient needs
+   none of the range scaling of the IEEE division sequence (|a| and |b| in
+   [2^-100, 2^100], normal quotient): the same reciprocal, Newton step and two
+   FMA corrections, without the scale and special-case fix-up steps that leave
+   such operands unchanged.  Equal to a / b for Spin's quotients (1 - g^2) /
+   (1 - g + 2 g xi), checked on the GPU against the division for every
+   denominator of several g. */
+__device__ __forceinline__ float div_unscaled(float a, float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  r = mcfl_fma(mcfl_fma(-b, r, 1.0f), r, r);
+  float q = a * r;
+  q = mcfl_fma(mcfl_fma(-b, q, a), r, q);
+  return mcfl_fma(mcfl_fma(-b, q, a), r, q);
+}
+
 /* Spin, CUDAMCFLtransport.cu:585-625. */
 __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint32_t a) {
   float xi = mcfl_rand_co(&x, a);
-  float t = mt.onemg2 / mcfl_fma(mt.twog, xi, mt.onemg);
+  float t = div_unscaled(mt.onemg2, mcfl_fma(mt.twog, xi, mt.onemg));
   float cost = mcfl_fma(-t, t, mt.onepg2) * mt.inv2g;
   if (mt.twog == 0.0f) cost = 2.0f * mcfl_rand_co(&x, a) - 1.0f;
-  const float s2 = mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)); /* sin^2 theta: 0 or >= 2^-23 */
+  const float s2 = mcfl_clamp0(mcfl_fma(-cost, cost, 1.0f)); /* sin^2 theta: 0 or >= 2^-23 */
   float sps, cps;
   mcfl_rand_sincos(&x, a, &sps, &cps);
   const float cosp = sps, sinp = cps; /* the reference's __sincosf argument order */
   const float dx = p.dx, dy = p.dy, dz = p.dz;
-  const float q = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)); /* temp^2 = 1 - dz^2: 0 or >= 6e-8 */
+  const float q = mcfl_clamp0(mcfl_fma(-dz, dz, 1.0f)); /* temp^2 = 1 - dz^2: 0 or >= 6e-8 */
   float nx, ny, nz;
   if (q == 0.0f) { /* the reference's temp == 0 test */
     const float sint = mcfl_sqrt_rs(s2);
@@ -170,16 +185,18 @@ __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint3
     ny = sint * sinp;
     nz = mcfl_copysign(cost, dz * cost);
   } else {
-    /* sint/temp and sint*temp from one reciprocal root of sint^2 temp^2 */
+    /* sint/temp and sint*temp from one reciprocal root of sint^2 temp^2;
+       nx = dx (cost + stt cosp dz) - dy stt sinp, ny = dy (...) + dx stt sinp */
     const float pq = s2 * q;
     const float r = pq > 0.0f ? mcfl_rsqrt(pq) : 0.0f;
     const float stt = s2 * r; /* sint / temp */
     const float stq = pq * r; /* sint * temp */
-    nx = mcfl_fma(stt, dx * dz * cosp - dy * sinp, dx * cost);
-    ny = mcfl_fma(stt, dy * dz * cosp + dx * sinp, dy * cost);
+    const float c = cost + (stt * cosp) * dz, w = stt * sinp;
+    nx = mcfl_fma(dx, c, -(dy * w));
+    ny = mcfl_fma(dy, c, dx * w);
     nz = mcfl_fma(-stq, cosp, dz * cost);
   }
-  const float r = mcfl_renorm(nx * nx + ny * ny + nz * nz);
+  const float r = mcfl_renorm(mcfl_fma(nz, nz, mcfl_fma(ny, ny, nx * nx)));
   p.dx = nx * r;
   p.dy = ny * r;
   p.dz = nz * r;
@@ -307,11 +324,16 @@ __device__ void exit_tally(const JobDev& J, const ExitDev& E, const Ph& p, bool 
                            unsigned long long* st) {
   lds_add_u64(&st[top ? MCFL_DT_R : MCFL_DT_T], p.w);
   if (mcfl_fabs(p.x - E.x0) < E.size_x && mcfl_fabs(p.y - E.y0) < E.size_y) {
-    const uint32_t iy = mcfl_f2u_rz((p.y - E.y0 + E.size_y) / E.dy);
-    const uint32_t ix = mcfl_f2u_rz((p.x - E.x0 + E.size_x) / E.dx);
+    /* the float quotients (p - origin + half size) / pixel size as the double
+       product with the pixel size's reciprocal, rounded to float: within
+       2^-52 of the quotient, which is 2^-49 or farther from every rounding
+       boundary of a float it is not equal to, so the same float */
+    const uint32_t iy = mcfl_f2u_rz((float)((double)(p.y - E.y0 + E.size_y) * E.rdy));
+    const uint32_t ix = mcfl_f2u_rz((float)((double)(p.x - E.x0 + E.size_x) * E.rdx));
     const uint64_t idx = (uint64_t)iy * (uint64_t)E.nx + ix;
     if (idx < E.npix) {
-      unsigned long long inc = scaled ? (unsigned long long)((double)p.w * escale + 0.5) : (unsigned long long)p.w;
+      unsigned long long inc = p.w;
+      if (__ballot(scaled) != 0ull && scaled) inc = (unsigned long long)((double)p.w * escale + 0.5);
       atomic_add_u64((top ? E.Rd : E.Tt) + idx, inc);
       if (E.flags & MCFL_F_SQ)
         atomic_add_u64((top ? E.Rd2 : E.Tt2) + idx, ((unsigned long long)p.w * (unsigned long long)p.w) >> 32);
@@ -380,6 +402,8 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     e.npix = J.det.npix;
     e.flags = J.flags;
     e.pad = 0u;
+    e.rdx = 1.0 / (double)J.det.dx;
+    e.rdy = 1.0 / (double)J.det.dy;
     e.Rd = J.Rd;
     e.Tt = J.Tt;
     e.Rd2 = J.Rd2;
@@ -433,6 +457,12 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
      when it leaves the voxel or dies.  Integer sums are unchanged. */
   uint32_t fv = 0xFFFFFFFFu, av = 0xFFFFFFFFu, ft = 0xFFFFFFFFu;
   unsigned long long facc = 0ull, aacc = 0ull;
+  /* A flush of the hitting density to global memory waits in (pv, pa) for the
+     start of the lane's next step: the step's label gather waits for all of
+     the wave's earlier memory operations, so an atomic issued just before
+     that wait would stall the wave for its whole latency. */
+  uint32_t pv = 0u;
+  unsigned long long pa = 0ull;
   /* the lane's last label lookup (voxel, label); cleared for each photon, whose
      medium may differ from the last one's */
   uint32_t lv = 0xFFFFFFFFu, ll = 0u;
@@ -592,6 +622,14 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
        medium) or MCd :79-226 (layered slab with one spherical inclusion) ---- */
     const MatDev* mats = smat + mbase;
     unsigned long long* st = stot + job * MCFL_NTOT;
+    /* the previous step's flush, issued once the step's material is read (a
+       relaunched photon's entry label may still be loading before that) */
+    auto flush_pending = [&]() {
+      if (pa != 0ull) {
+        atomic_add_u64(fhd + pv, pa);
+        pa = 0ull;
+      }
+    };
     float s;
     uint32_t nb = p.lab;
     MatDev mm; /* material of the absorption and Spin */
@@ -605,10 +643,12 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       if ((flags & MCFL_F_FHD) && in) {
         if (vi != fv) {
           if (facc != 0ull) {
-            if (ft != 0xFFFFFFFFu)
+            if (ft != 0xFFFFFFFFu) {
               lds_add_u64(stile + ft, facc);
-            else
-              atomic_add_u64(fhd + fv, facc);
+            } else {
+              pv = fv;
+              pa = facc;
+            }
           }
           fv = vi;
           facc = 0ull;
@@ -625,6 +665,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       const uint32_t L = p.lab;
       const float zlo = szb[2u * L], zhi = szb[2u * L + 1u];
       const MatDev ml = mats[L];
+      flush_pending();
       if (in_sphere(m, p.x, p.y, p.z)) { /* :83-92 */
         const float mutr = mats[m.n_layers + 2u].mutr;
         s = (mutr != kFltMax) ? -mcfl_log(mcfl_rand_oc(&x, a)) * mutr : 2.0f;
@@ -657,6 +698,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       mm = in_sphere(m, p.x, p.y, p.z) ? mats[m.n_layers + 2u] : mats[p.lab]; /* :200-224 */
     } else {
       const MatDev mc = mats[p.lab];
+      flush_pending();
       if (mc.mutr != kFltMax) {
         s = -mcfl_log(mcfl_rand_oc(&x, a)) * mc.mutr;
         if (mcfl_fma(s, p.dz, p.z) < 0.0f) {
@@ -761,6 +803,10 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           atomic_add_u64(fhd + fv, facc);
       }
       ft = 0xFFFFFFFFu;
+      if (pa != 0ull) {
+        atomic_add_u64(fhd + pv, pa);
+        pa = 0ull;
+      }
       if (aacc != 0ull) atomic_add_u64(axyz + av, aacc);
       fv = av = 0xFFFFFFFFu;
       facc = aacc = 0ull;
@@ -877,6 +923,66 @@ __global__ void mcfl_accumulate_copies_kernel(unsigned long long* p, uint64_t n,
    such calls on one GPU must not overlap. */
 static __device__ __align__(256) unsigned long long scratch[MCFL_NA_BYTES / 8];
 
+/* ---- self-checks of the per-step arithmetic (test support) ----
+   Each compares a form the transport kernel uses with an independent one on
+   the GPU itself, over a whole range of inputs, counting mismatches. */
+
+/* float -> uint32 toward zero from the bits alone (no conversion instruction) */
+__device__ __forceinline__ uint32_t f2u_rz_bits(uint32_t b) {
+  if (b >> 31) return 0u; /* negative or -0 (and NaN with the sign bit) */
+  const uint32_t e = b >> 23, m = (b & 0x007FFFFFu) | 0x00800000u;
+  if (e == 255u) return (b & 0x007FFFFFu) ? 0u : 0xFFFFFFFFu; /* NaN, +inf */
+  if (e >= 159u) return 0xFFFFFFFFu;                           /* >= 2^32 */
+  if (e < 127u) return 0u;                                     /* < 1 */
+  return e >= 150u ? m << (e - 150u) : m >> (150u - e);
+}
+
+__global__ void mcfl_selfcheck_kernel(int which, uint32_t p, uint64_t lo, uint64_t hi, unsigned long long* bad,
+                                      unsigned long long* first) {
+  for (uint64_t i = lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = (uint32_t)i;
+    bool ok = true;
+    if (which == MCFL_SC_U2F_RZ) { /* instruction under toward-zero MODE vs the exact conversion of u's top 24 bits */
+      ok = mcfl_fbits(mcfl_u2f_rz(u)) == mcfl_fbits((float)(u & (0xFFFFFF00u >> __builtin_clz(u | 1u))));
+    } else if (which == MCFL_SC_F2U_RZ) { /* saturating conversion instruction vs the bits */
+      ok = mcfl_f2u_rz(mcfl_bitsf(u)) == f2u_rz_bits(u);
+    } else if (which == MCFL_SC_DIV) { /* Spin's quotient: numerator bits p, denominator bits u */
+      const float a = mcfl_bitsf(p), b = mcfl_bitsf(u);
+      ok = mcfl_fbits(div_unscaled(a, b)) == mcfl_fbits(a / b);
+    } else if (which == MCFL_SC_PIXEL) { /* exit-tally quotient: pixel size bits p, offset bits u */
+      const float d = mcfl_bitsf(p), q = mcfl_bitsf(u);
+      const double rd = 1.0 / (double)d;
+      ok = mcfl_fbits((float)((double)q * rd)) == mcfl_fbits(q / d);
+    }
+    if (!ok) {
+      atomicAdd(bad, 1ull);
+      atomicMin(first, (unsigned long long)i);
+    }
+  }
+}
+
+/* The shared FP32 functions on the GPU, for comparison with the host build. */
+__global__ void mcfl_math_eval_kernel(int fn, const uint32_t* in, uint32_t* out, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = in[i];
+    if (fn == MCFL_ME_LOG) {
+      out[i] = mcfl_fbits(mcfl_log(mcfl_bitsf(u)));
+    } else if (fn == MCFL_ME_RSQRT) {
+      out[i] = mcfl_fbits(mcfl_rsqrt(mcfl_bitsf(u)));
+    } else if (fn == MCFL_ME_RENORM) {
+      out[i] = mcfl_fbits(mcfl_renorm(mcfl_bitsf(u)));
+    } else if (fn == MCFL_ME_SINCOS) { /* of 2 pi times the [0,1) draw whose 32-bit value is u */
+      float s, c;
+      mcfl_sincos_quarter(mcfl_u2f_rz(u) * 0x1p-30f, &s, &c);
+      out[2 * i] = mcfl_fbits(s);
+      out[2 * i + 1] = mcfl_fbits(c);
+    } else if (fn == MCFL_ME_DRAW_OC) {
+      out[i] = mcfl_fbits(mcfl_fma(mcfl_u2f_rz(u), -0x1p-32f, 1.0f));
+    }
+  }
+}
+
 /* ------------------------------------------------------------------ host */
 
 namespace {
@@ -973,6 +1079,44 @@ extern "C" int mcfl_accumulate_copies(unsigned long long* p, uint64_t n, uint32_
   if (blocks > 8192) blocks = 8192;
   mcfl_accumulate_copies_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(p, n, ncopies, acc);
   return (int)hipGetLastError();
+}
+
+extern "C" int mcfl_selfcheck(int which, uint32_t p, uint64_t lo, uint64_t hi, uint64_t* mismatches,
+                              uint64_t* first) {
+  if (which < 0 || which > MCFL_SC_PIXEL || lo > hi || hi > (1ull << 32) || !mismatches || !first) return -1;
+  unsigned long long* d = nullptr;
+  if (hipMalloc((void**)&d, 16) != hipSuccess) return -1;
+  const unsigned long long init[2] = {0ull, ~0ull};
+  hipError_t e = hipMemcpy(d, init, 16, hipMemcpyHostToDevice);
+  if (e == hipSuccess && hi > lo) {
+    mcfl_selfcheck_kernel<<<dim3(4096), dim3(256)>>>(which, p, lo, hi, d, d + 1);
+    e = hipGetLastError();
+  }
+  unsigned long long res[2] = {0ull, 0ull};
+  if (e == hipSuccess) e = hipMemcpy(res, d, 16, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return (int)e;
+  *mismatches = res[0];
+  *first = res[1];
+  return 0;
+}
+
+extern "C" int mcfl_math_eval(int fn, const uint32_t* in, uint32_t* out, uint64_t n) {
+  if (fn < 0 || fn > MCFL_ME_DRAW_OC || !in || !out) return -1;
+  if (n == 0) return 0;
+  const uint64_t nout = fn == MCFL_ME_SINCOS ? 2 * n : n;
+  uint32_t *din = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc((void**)&din, 4 * n);
+  if (e == hipSuccess) e = hipMalloc((void**)&dout, 4 * nout);
+  if (e == hipSuccess) e = hipMemcpy(din, in, 4 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    mcfl_math_eval_kernel<<<dim3(4096), dim3(256)>>>(fn, din, dout, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 4 * nout, hipMemcpyDeviceToHost);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return e == hipSuccess ? 0 : (int)e;
 }
 
 extern "C" int mcfl_fill_u64(unsigned long long* p, uint64_t n, hipStream_t stream) {

@@ -176,30 +176,32 @@ static void orc_spin(const mat_t* m, orc_photon* p, uint64_t* x, uint32_t a) {
   float temp = m->onemg2 / mcfl_fma(m->twog, xi, m->onemg);
   float cost = mcfl_fma(-temp, temp, m->onepg2) * m->inv2g;
   if (m->g == 0.0f) cost = 2.0f * mcfl_rand_co(x, a) - 1.0f;
-  float sin2t = mcfl_fmax(0.0f, mcfl_fma(-cost, cost, 1.0f)); /* sint^2 (:600) */
+  float sin2t = mcfl_clamp0(mcfl_fma(-cost, cost, 1.0f)); /* sint^2 (:600) */
   float s_psi, c_psi;
   mcfl_rand_sincos(x, a, &s_psi, &c_psi);
   float cosp = s_psi; /* :603 assigns the sine to cosp */
   float sinp = c_psi;
   float dx = p->dx, dy = p->dy, dz = p->dz;
-  float temp2 = mcfl_fmax(0.0f, mcfl_fma(-dz, dz, 1.0f)); /* temp^2, temp = sqrt(1 - dz^2) (:605) */
+  float temp2 = mcfl_clamp0(mcfl_fma(-dz, dz, 1.0f)); /* temp^2, temp = sqrt(1 - dz^2) (:605) */
   float ndx, ndy, ndz;
   if (temp2 == 0.0f) { /* :607-612 normal incidence */
     float sint = mcfl_sqrt_rs(sin2t);
     ndx = sint * cosp;
     ndy = sint * sinp;
     ndz = mcfl_copysign(cost, dz * cost);
-  } else { /* :613-618, with sint/temp and sint*temp from one reciprocal root of sint^2 temp^2 */
+  } else { /* :613-618, with sint/temp and sint*temp from one reciprocal root of sint^2 temp^2,
+              grouped as dx (cost + st cosp dz) - dy st sinp, dy (cost + st cosp dz) + dx st sinp */
     float pq = sin2t * temp2;
     float r = pq > 0.0f ? mcfl_rsqrt(pq) : 0.0f;
     float st = sin2t * r;
     float stq = pq * r;
-    ndx = mcfl_fma(st, dx * dz * cosp - dy * sinp, dx * cost);
-    ndy = mcfl_fma(st, dy * dz * cosp + dx * sinp, dy * cost);
+    float c = cost + (st * cosp) * dz, w = st * sinp;
+    ndx = mcfl_fma(dx, c, -(dy * w));
+    ndy = mcfl_fma(dy, c, dx * w);
     ndz = mcfl_fma(-stq, cosp, dz * cost);
   }
   /* :621-624 renormalisation with a reciprocal square root (first order near 1) */
-  float r = mcfl_renorm(ndx * ndx + ndy * ndy + ndz * ndz);
+  float r = mcfl_renorm(mcfl_fma(ndz, ndz, mcfl_fma(ndy, ndy, ndx * ndx)));
   p->dx = ndx * r;
   p->dy = ndy * r;
   p->dz = ndz * r;

@@ -97,6 +97,33 @@ def test_tof_quotient_is_the_ieee_division():
     assert O.tof_div_mismatches(0xC0000000, 0xC0000000 + (1 << 26)) == 0
 
 
+def test_pixel_quotient_as_double_product():
+    """The kernel computes the exit tally's pixel quotient q / d
+    (CUDAMCFLtransport.cu:374-375,401-402) as the double product with the double
+    reciprocal of d, rounded to float.  Its error (2^-52 relative) is below the
+    2^-49 relative distance a float quotient keeps from every float midpoint
+    it does not equal, so the float is the IEEE quotient: checked on 4M random
+    pairs and on 1M quotients built to fall within an ulp of a midpoint."""
+    rng = np.random.default_rng(3)
+    n = 1 << 22
+    d = (10.0 ** rng.uniform(-4, 1, n)).astype(np.float32)
+    q = (d * rng.uniform(0, 2000, n)).astype(np.float32)
+    got = (q.astype(np.float64) * (1.0 / d.astype(np.float64))).astype(np.float32)
+    assert np.array_equal(got, q / d)
+    # near midpoints: m = (2k+1) 2^(e-24), q = float(m d) puts q / d next to m
+    m = n // 4
+    dd = (10.0 ** rng.uniform(-3, 0, m)).astype(np.float32)
+    k = rng.integers(1 << 23, 1 << 24, m).astype(np.float64)
+    e = rng.integers(-2, 12, m).astype(np.float64)
+    mid = (2 * k + 1) * 2.0 ** (e - 24)
+    qq = (mid * dd.astype(np.float64)).astype(np.float32)
+    got = (qq.astype(np.float64) * (1.0 / dd.astype(np.float64))).astype(np.float32)
+    assert np.array_equal(got, qq / dd)
+    # the planted mistake: the float reciprocal instead of the double one differs
+    bad = (qq * (np.float32(1) / dd)).astype(np.float32)
+    assert not np.array_equal(bad, qq / dd)
+
+
 def test_u32_to_float_toward_zero():
     rng = np.random.default_rng(2)
     u = np.concatenate([rng.integers(0, 2 ** 32, 100000, dtype=np.uint64),
