@@ -112,7 +112,8 @@ struct ExitDev {
   unsigned long long* Tt;
   unsigned long long* Rd2;
   unsigned long long* Tt2;
-  double rdx, rdy; /* 1/dx, 1/dy rounded to double */
+  double rdx, rdy;   /* 1/dx, 1/dy rounded to double */
+  double exit_scale; /* the job's image increment scale (MCFL_F_SCALED) */
 };
 
 struct JobDev {

@@ -404,6 +404,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     e.pad = 0u;
     e.rdx = 1.0 / (double)J.det.dx;
     e.rdy = 1.0 / (double)J.det.dy;
+    e.exit_scale = J.exit_scale;
     e.Rd = J.Rd;
     e.Tt = J.Tt;
     e.Rd2 = J.Rd2;
@@ -556,7 +557,6 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           flags = J.flags;
           fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
           axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
-          escale = J.exit_scale;
           if (J.src_kind == MCFL_SRC_EMISSION) {
             /* importance-sampled emission voxel (Walker alias table), then an
                isotropic photon from its corner as in CUDAMCFLmain.cu:481-487 */
@@ -721,10 +721,11 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       cl = cell_in(m, p.x, p.y, p.z);
       vi = in ? cell_index(m, cl) : 0u;
       /* the end point's label (:319-333): gathered unless it is the voxel of
-         the lane's last lookup; the hitting-density update, which does not
-         depend on it, runs while the gather is in flight */
+         the lane's last lookup (the temporal mixed-media variant, short of
+         registers, keeps no such cache); the hitting-density update, which
+         does not depend on it, runs while the gather is in flight */
       const bool look = nb == p.lab && mc.mutr != kFltMax;
-      const bool gather = look && in && vi != lv;
+      const bool gather = look && in && (TEMPORAL && PERJOB ? true : vi != lv);
       uint32_t got = 0u;
       if (gather) got = (uint32_t)m.labels[vi];
       fhd_visit();
@@ -733,8 +734,10 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           nb = 1u;
         } else if (gather) {
           nb = got;
-          lv = vi;
-          ll = got;
+          if (!(TEMPORAL && PERJOB)) {
+            lv = vi;
+            ll = got;
+          }
         } else {
           nb = ll;
         }
@@ -749,7 +752,9 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     if constexpr (LAYERED) fhd_visit();
 
     if (p.lab == 0u || p.lab == m.last) {
-      exit_tally<TEMPORAL>(A.jobs[job], sexit[job], p, p.lab == 0u, (flags & MCFL_F_SCALED) != 0u, escale, st);
+      /* a job's image scale from its exit constants in LDS; a voxel emission job's per voxel */
+      exit_tally<TEMPORAL>(A.jobs[job], sexit[job], p, p.lab == 0u, (flags & MCFL_F_SCALED) != 0u,
+                           WORK == MCFL_WORK_JOBS ? sexit[job].exit_scale : escale, st);
       p.w = 0u;
       s = 0.0f;
     }

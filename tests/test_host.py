@@ -347,6 +347,21 @@ def test_integration_example_compiles_and_links(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+def test_no_kernel_variant_spills(tmp_path):
+    """Every transport kernel variant fits its vector register budget without
+    spilling to scratch memory (compiled for gfx950 here; the counts come
+    from the code object metadata of the device assembly).  Scalar register
+    spills, which go to lanes of a vector register, are allowed."""
+    out = str(tmp_path)
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "cudamcfl_amd", "csrc"), f"OUT={out}", "asm"], check=True,
+                   capture_output=True, timeout=600)
+    text = open(os.path.join(out, "mcfl_transport.s")).read()
+    kernels = re.findall(r"\.name:\s+(_Z\w*mcfl_transport_kernel\w*)(.*?)\.vgpr_spill_count:\s+(\d+)", text, flags=re.S)
+    assert len(kernels) >= 20
+    spills = {name: int(n) for name, _, n in kernels if int(n) != 0}
+    assert not spills, spills
+
+
 def test_bench_refuses_more_than_one_gpu():
     """bench.py measures one GPU; --gpus N above 1 exits with a message before
     touching a device."""
