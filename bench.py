@@ -40,11 +40,13 @@ def f_step(temporal, fhd):
     return F_STEP_REF - (0 if temporal else F_TOF) - (0 if fhd else F_FHD)
 
 
-# MI355X FP32 VALU issue rate: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz lane slots/s
-# (the published 157.3 TFLOP/s FP32 vector figure / 2 for FMA / 2 for packed
-# FP32, which this build does not use); one wave64 FP32 instruction per 4 cycles
-# per SIMD, confirmed by an FMA-chain calibration.
-PEAK_SLOTS = 256 * 4 * 16 * 2.4e9
+# MI355X VALU issue ceiling in lane slots/s: the published 157.3 TFLOP/s FP32
+# vector figure / 2 flops per FMA = 256 CU x 4 SIMD x 32 lanes x 2.4 GHz, one
+# wave64 instruction per 2 cycles per SIMD.  Streams of independent FP32 or
+# integer instructions with vector-register operands issue at 2.4-2.8 cycles
+# per wave64 instruction on the GPU (probe), below it; an operand in a scalar
+# register costs 4.4 cycles when every instruction has one.
+PEAK_SLOTS = 256 * 4 * 32 * 2.4e9
 CPU_THREADS_MAX = 16  # the host CPU share of one GPU on the measuring boxes
 
 
@@ -132,17 +134,18 @@ def cpu_baseline(target_s=20.0):
                              "single_thread_photons_per_s": m1 / dtm1}}
 
 
-def measured_traffic(photons):
-    """HBM bytes per launch of this workload from the committed counter
-    profile (FETCH_SIZE and WRITE_SIZE, each scaled on a plain copy); None when
-    the profile covers another photon count."""
+def measured_counters(photons):
+    """Per-launch counters of this workload from the committed profile: HBM
+    bytes (FETCH_SIZE and WRITE_SIZE, each scaled on a plain copy) and VALU
+    wave-instructions issued (SQ_INSTS_VALU); None when the profile covers
+    another photon count."""
     path = os.path.join(ROOT, "profiles", "mcfl_transport_kernel_128x128x64_1e8_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t["bytes_per_launch"] if t.get("photons") == photons else None
+    return t if t.get("photons") == photons else None
 
 
 def copy_probe(nbytes):
@@ -525,13 +528,19 @@ def main():
     F = f_step(temporal=False, fhd=True)
     achieved = steps_per_s * F / 1e12
     peak = PEAK_SLOTS / 1e12
-    traffic = measured_traffic(N)
+    prof = measured_counters(N)
+    traffic = prof["bytes_per_launch"] if prof else None
     ms_step = sum(kms) / args.steps
     # the secondary roofline: HBM bytes per launch over the launch time, against 8 TB/s
-    hbm = None
+    hbm = issued = None
     if traffic:
         gbs = traffic / (ms_step * 1e-3) / 1e9
         hbm = {"achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0}
+    if prof and prof.get("valu_insts"):
+        # lane slots the kernel issues (VALU wave-instructions x 64) over this launch's time: at most the ceiling
+        ts = prof["valu_insts"] * 64 / (ms_step * 1e-3) / 1e12
+        issued = {"achieved": ts, "frac": ts / peak, "slots_per_step": prof["valu_insts"] * 64 / prof["steps"],
+                  "source": "SQ_INSTS_VALU of the committed counter profile, over this run's launch time"}
     out = {
         "metric": "photons/s at 128x128x64 voxels; p50/p99 sim latency, batched vs plain",
         "value": value,
@@ -553,15 +562,16 @@ def main():
         "photon_steps_per_s": steps_per_s,
         "steps_per_photon": steps / (N * args.steps),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tslot/s",
-                     "frac": achieved / peak, "traffic": traffic, "hbm": hbm,
+                     "frac": achieved / peak, "traffic": traffic, "hbm": hbm, "issued": issued,
                      "F_step": F, "F_step_reference": F_STEP_REF,
-                     "ceiling": "FP32 VALU issue, 256 CU x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 Tslot/s (157.3 TFLOP/s "
-                                "FP32 vector / 2 FMA / 2 packed), one wave64 FP32 instruction per 4 cycles per SIMD "
-                                "(FMA-chain calibration)",
-                     "note": f"{F} VALU issue slots per photon-step (the reference loop's {F_STEP_REF} less the "
-                             f"time-of-flight update this launch does not need: no temporal detectors) x photon-steps "
-                             "/ kernel time; traffic = HBM bytes per launch from FETCH_SIZE/WRITE_SIZE scaled on a "
-                             "copy (profiles/)"},
+                     "ceiling": "VALU issue, 256 CU x 4 SIMD x 32 lanes x 2.4 GHz = 78.6 Tslot/s (the published "
+                                "157.3 TFLOP/s FP32 vector figure / 2 flops per FMA): one wave64 instruction per 2 "
+                                "cycles per SIMD",
+                     "note": f"achieved = {F} algorithmic VALU slots per photon-step (the reference loop's "
+                             f"{F_STEP_REF} less the time-of-flight update this launch does not need: no temporal "
+                             "detectors) x photon-steps / kernel time; issued = the slots the kernel actually issues "
+                             "(divergent branches, addresses, 64-bit tallies); traffic = HBM bytes per launch from "
+                             "FETCH_SIZE/WRITE_SIZE scaled on a copy (profiles/)"},
         "gpu": info,
     }
     if not args.plain_only:
