@@ -914,6 +914,7 @@ struct mcfl_server {
   BufPool pool;    // declared before the requests, so it outlives their tallies
   HostPool hpool;  // pinned staging of the copy-outs
   uint64_t epoch_target = 20000000;  // photons per epoch to aim for (env MCFL_EPOCH_TARGET)
+  double oldest_share = 0.75;        // of an epoch's photons above the floors, the oldest request's (env MCFL_OLDEST_SHARE)
   int reserve = 8;                   // workgroup slots an epoch leaves free (env MCFL_SERVER_RESERVE)
   std::vector<std::unique_ptr<Medium>> media;
   std::deque<std::unique_ptr<Request>> queue;
@@ -960,6 +961,7 @@ extern "C" mcfl_status mcfl_server_create(mcfl_ctx* c, uint64_t budget, uint64_t
   s->pool.limit = budget;  // recycled buffers kept: at most the admission budget again
   s->epoch_photons = epoch_photons;
   if (const char* e = getenv("MCFL_EPOCH_TARGET")) s->epoch_target = strtoull(e, nullptr, 10);
+  if (const char* e = getenv("MCFL_OLDEST_SHARE")) s->oldest_share = std::min(1.0, std::max(0.0, atof(e)));
   if (const char* e = getenv("MCFL_SERVER_RESERVE")) s->reserve = std::max(0, atoi(e));
   if (hipSetDevice(c->device) != hipSuccess) {
     delete s;
@@ -1137,11 +1139,14 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
   const auto t_build0 = std::chrono::steady_clock::now();
 
   // 1. Launch the next epoch if its slot is free.  An epoch carries about
-  // epoch_target photons (about 45 ms of device time): every active request
-  // with photons left in its phase gets at least epoch_photons of them, and
-  // the rest of the target goes to the requests with the least work left
-  // first (shortest remaining work first), so small requests finish in the
-  // next epoch while large ones keep advancing.
+  // epoch_target photons (tens of milliseconds of device time): every active request
+  // with photons left in its phase gets at least epoch_photons of them; of
+  // the rest of the target the oldest request (first submitted) takes the
+  // fraction oldest_share and the remainder goes to the requests with the
+  // least work left first (shortest remaining work first).  Small requests
+  // finish in the next epoch while the oldest one runs at most of the GPU's
+  // rate, so a large request is not starved behind a stream of small ones
+  // (shortest-first alone leaves it last whenever anything smaller is open).
   bool launched = false;
   Epoch& E = s->ep[s->next];
   std::vector<Request*> open;
@@ -1154,13 +1159,25 @@ extern "C" mcfl_status mcfl_server_step(mcfl_server* s, int64_t* done_ids, int32
   std::map<int64_t, uint64_t> share;
   {
     uint64_t floors = 0;
-    for (const Request* q : open) floors += std::min<uint64_t>(q->r.photons - q->done, s->epoch_photons);
-    uint64_t extra = s->epoch_target > floors ? s->epoch_target - floors : 0;
+    const Request* oldest = nullptr;
     for (const Request* q : open) {
-      const uint64_t rem = q->r.photons - q->done, fl = std::min<uint64_t>(rem, s->epoch_photons);
-      const uint64_t add = std::min<uint64_t>(rem - fl, extra);
+      const uint64_t fl = std::min<uint64_t>(q->r.photons - q->done, s->epoch_photons);
+      floors += fl;
+      share[q->id] = fl;
+      if (!oldest || q->id < oldest->id) oldest = q;
+    }
+    uint64_t extra = s->epoch_target > floors ? s->epoch_target - floors : 0;
+    if (oldest) {
+      const uint64_t rem = oldest->r.photons - oldest->done - share[oldest->id];
+      const uint64_t add = std::min<uint64_t>(rem, (uint64_t)(s->oldest_share * (double)extra));
+      share[oldest->id] += add;
       extra -= add;
-      share[q->id] = fl + add;
+    }
+    for (const Request* q : open) {
+      const uint64_t rem = q->r.photons - q->done - share[q->id];
+      const uint64_t add = std::min<uint64_t>(rem, extra);
+      extra -= add;
+      share[q->id] += add;
     }
   }
   if (!E.busy) {

@@ -392,8 +392,9 @@ typedef struct mcfl_request_result {
 } mcfl_request_result;
 
 /* Each epoch is one launch of about 2e7 photons (env MCFL_EPOCH_TARGET): every
-   active request gets at least epoch_photons of its phase, the rest of the
-   target goes to the requests with the least work left first.  Admission is
+   active request gets at least epoch_photons of its phase; of the rest of the
+   target the oldest request takes 3/4 (env MCFL_OLDEST_SHARE) and the others
+   share what remains, least work left first.  Admission is
    FIFO within mem_budget_bytes.  Two epochs are in flight at a time on two
    streams; an epoch leaves 8 workgroup slots free (env MCFL_SERVER_RESERVE)
    so that finished requests' tallies are copied out beside it. */
