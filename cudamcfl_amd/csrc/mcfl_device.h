@@ -38,6 +38,9 @@
 #endif
 
 enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2, MCFL_WORK_VLANES = 3 };
+/* Kernel-internal: a MCFL_WORK_JOBS launch of one job (the job's tally
+   pointers, flags and material base are then wave-uniform scalars). */
+enum { MCFL_WORK_ONE = 4 };
 enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2, MCFL_SRC_EMISSION = 3 };
 enum {
   MCFL_DT_LAUNCHED_W = 0,
@@ -52,18 +55,18 @@ enum {
   MCFL_DT_ABORTED
 };
 
-/* Per-material constants (48 bytes, LDS resident).  twog == 0 <=> g == 0.
+/* Per-material constants (48 bytes, LDS resident).  inv2g == 0 <=> g == 0.
    The surface quotients are the two divisions of Reflect against the media
    above (material 0) and below (the table's last outer material), rounded as
    the kernel would round them (n*n)/(n'*n') and n/n'. */
-struct MatDev {
+struct alignas(16) MatDev {
   float mutr;     /* 1/(mua_base + mus), FLT_MAX for glass */
   float mua_mutr; /* mua*mutr: absorbed fraction per interaction */
   float n;        /* refractive index */
   float onemg;    /* 1 - g */
   float onemg2;   /* 1 - g^2 */
   float onepg2;   /* 1 + g^2 */
-  float twog;     /* 2g */
+  float twog32;   /* 2g * 2^-32: Spin's coefficient of the raw 32-bit draw */
   float inv2g;    /* 1/(2g), 0 when g == 0 */
   float q_top, r_top; /* (n*n)/(n0*n0), n/n0 */
   float q_bot, r_bot; /* the same against the medium below */
@@ -221,8 +224,13 @@ static inline uint32_t mcfl_wg_of(const LaunchArgs* A) { return A->wg ? A->wg : 
 
 /* Dynamic LDS of a launch: per-job totals, fhd tile, material table, layer
    bounds, per-job exit constants. */
+/* fhd tile entries rounded up to an even count: the material table after the
+   tile starts on a 16-byte boundary (the kernel reads a material's fields
+   with 16-byte loads). */
+#define MCFL_TILE_SLOTS(tile_n) (((tile_n) + 1u) & ~1u)
+
 static inline uint32_t mcfl_lds_bytes(uint32_t nslots, uint32_t tile_n, uint32_t n_mats, uint32_t n_zb) {
-  return (nslots * MCFL_NTOT + tile_n) * 8u + n_mats * (uint32_t)sizeof(MatDev) + n_zb * 8u +
+  return (nslots * MCFL_NTOT + MCFL_TILE_SLOTS(tile_n)) * 8u + n_mats * (uint32_t)sizeof(MatDev) + n_zb * 8u +
          nslots * (uint32_t)sizeof(ExitDev);
 }
 

@@ -54,7 +54,7 @@ static void mat_of(float mutr, float mua, float g, float n, MatDev& m) {
   m.onemg = 1.0f - g;
   m.onemg2 = 1.0f - g2;
   m.onepg2 = 1.0f + g2;
-  m.twog = 2.0f * g;
+  m.twog32 = 2.0f * g * 0x1p-32f;
   m.inv2g = (g != 0.0f) ? 1.0f / (2.0f * g) : 0.0f;
 }
 

@@ -166,12 +166,38 @@ __device__ __forceinline__ float div_unscaled(float a, float b) {
   return mcfl_fma(mcfl_fma(-b, q, a), r, q);
 }
 
+/* A material from the LDS table: its first 32 bytes (the fields of the step,
+   the absorption and Spin) as two 16-byte loads; the table is 16-byte
+   aligned. */
+__device__ __forceinline__ MatDev load_mat(const MatDev* t) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4* q = reinterpret_cast<const f4*>(t);
+  const f4 a = q[0], b = q[1];
+  MatDev r;
+  r.mutr = a.x;
+  r.mua_mutr = a.y;
+  r.n = a.z;
+  r.onemg = a.w;
+  r.onemg2 = b.x;
+  r.onepg2 = b.y;
+  r.twog32 = b.z;
+  r.inv2g = b.w;
+  r.q_top = t->q_top;
+  r.r_top = t->r_top;
+  r.q_bot = t->q_bot;
+  r.r_bot = t->r_bot;
+  return r;
+}
+
 /* Spin, CUDAMCFLtransport.cu:585-625. */
 __device__ __forceinline__ void spin(const MatDev& mt, Ph& p, uint64_t& x, uint32_t a) {
-  float xi = mcfl_rand_co(&x, a);
-  float t = div_unscaled(mt.onemg2, mcfl_fma(mt.twog, xi, mt.onemg));
+  /* 1 - g + 2g xi with xi = U 2^-32 (U the draw converted toward zero) as
+     1 - g + (2g 2^-32) U: the same real product, so the same FMA, wherever
+     2g 2^-32 is exact (|2g| >= 2^-94); below that both round to 1 - g = 1 */
+  const float u = mcfl_u2f_rz(mcfl_mwc_next(&x, a));
+  float t = div_unscaled(mt.onemg2, mcfl_fma(mt.twog32, u, mt.onemg));
   float cost = mcfl_fma(-t, t, mt.onepg2) * mt.inv2g;
-  if (mt.twog == 0.0f) cost = 2.0f * mcfl_rand_co(&x, a) - 1.0f;
+  if (mt.inv2g == 0.0f) cost = 2.0f * mcfl_rand_co(&x, a) - 1.0f; /* g == 0 */
   const float s2 = mcfl_clamp0(mcfl_fma(-cost, cost, 1.0f)); /* sin^2 theta: 0 or >= 2^-23 */
   float sps, cps;
   mcfl_rand_sincos(&x, a, &sps, &cps);
@@ -381,11 +407,14 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   /* dynamic LDS: [per-job totals | fhd tile | materials | layer bounds | per-job exit constants] */
   extern __shared__ __align__(16) unsigned long long lds[];
 
+  /* JOBS: photons of the job table (ONE: a table of one job) */
+  constexpr bool JOBS = WORK == MCFL_WORK_JOBS || WORK == MCFL_WORK_ONE;
+  constexpr bool ONE = WORK == MCFL_WORK_ONE;
   const uint32_t tid = threadIdx.x;
   const uint32_t nslots = (WORK == MCFL_WORK_JOBS) ? A.njobs : 1u;
   unsigned long long* stot = lds;
   unsigned long long* stile = lds + nslots * MCFL_NTOT;
-  MatDev* smat = reinterpret_cast<MatDev*>(stile + A.tile_n);
+  MatDev* smat = static_cast<MatDev*>(__builtin_assume_aligned(stile + MCFL_TILE_SLOTS(A.tile_n), 16));
   float* szb = reinterpret_cast<float*>(smat + A.n_mats);
   ExitDev* sexit = reinterpret_cast<ExitDev*>(szb + (LAYERED ? 2u * (A.med.n_layers + 2u) : 0u));
   for (uint32_t i = tid; i < A.n_mats; i += WGS) smat[i] = A.mats[i];
@@ -480,7 +509,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   const uint32_t tile_job = __builtin_amdgcn_readfirstlane(swg[1]);
   const uint32_t tx0 = __builtin_amdgcn_readfirstlane(swg[2]), ty0 = __builtin_amdgcn_readfirstlane(swg[3]);
   /* wave-uniform chunk of the current job's photons: [pool_next, pool_next + pool_left) */
-  uint32_t cur = __builtin_amdgcn_readfirstlane(swg[0]), pool_left = 0;
+  uint32_t cur = ONE ? 0u : __builtin_amdgcn_readfirstlane(swg[0]), pool_left = 0;
   /* PERJOB: the wave's photons share one medium (m stays wave-uniform), that
      of job wmed; a wave moving on to a job in another medium lets its photons
      finish first */
@@ -489,7 +518,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   uint64_t pool_next = 0;
   bool pool_dead = false;
 
-  if (WORK == MCFL_WORK_VOXELS || WORK == MCFL_WORK_VLANES) {
+  if (ONE || WORK == MCFL_WORK_VOXELS || WORK == MCFL_WORK_VLANES) {
     mbase = A.jobs[0].mat_base;
     flags = A.jobs[0].flags;
     fhd = A.jobs[0].fhd ? A.jobs[0].fhd + xcc * A.jobs[0].copy_stride : nullptr;
@@ -507,7 +536,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     while (needmask != 0ull) {
       if (pool_left == 0u) {
         if (pool_dead) break;
-        const uint64_t cnt = WORK == MCFL_WORK_JOBS ? A.jobs[cur].count : A.total;
+        const uint64_t cnt = JOBS ? A.jobs[cur].count : A.total;
         /* guided: the job's remainder (beyond the wave's last chunk) over
            four times the launch's waves, within [64, chunk_max] */
         const uint64_t rem = cnt > pool_next ? cnt - pool_next : 0ull;
@@ -544,19 +573,21 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
         alive = true;
         accA = 0;
         iters = 0;
-        if (WORK == MCFL_WORK_JOBS) {
+        if (JOBS) {
           /* the job table entry through the wave-uniform cur: scalar loads,
              which do not wait behind the wave's outstanding vector memory
              operations (tally atomics) */
-          job = cur;
-          const JobDev& J = A.jobs[cur];
+          const JobDev& J = A.jobs[ONE ? 0u : cur];
           const uint64_t k = g + J.k0;
           a = A.a_tab[k & A.a_mask];
           x = mcfl_photon_state(J.key, k, a);
-          mbase = J.mat_base;
-          flags = J.flags;
-          fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
-          axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
+          if (!ONE) {
+            job = cur;
+            mbase = J.mat_base;
+            flags = J.flags;
+            fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
+            axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
+          }
           if (J.src_kind == MCFL_SRC_EMISSION) {
             /* importance-sampled emission voxel (Walker alias table), then an
                isotropic photon from its corner as in CUDAMCFLmain.cu:481-487 */
@@ -642,14 +673,12 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     auto fhd_visit = [&]() {
       if ((flags & MCFL_F_FHD) && in) {
         if (vi != fv) {
-          if (facc != 0ull) {
-            if (ft != 0xFFFFFFFFu) {
-              lds_add_u64(stile + ft, facc);
-            } else {
-              pv = fv;
-              pa = facc;
-            }
-          }
+          /* the sum of the voxel left: into the tile, or pending (pa is 0
+             here: the step's start flushed it) */
+          const bool tiled = ft != 0xFFFFFFFFu;
+          if (tiled && facc != 0ull) lds_add_u64(stile + ft, facc);
+          pv = fv;
+          pa = tiled ? 0ull : facc;
           fv = vi;
           facc = 0ull;
           ft = 0xFFFFFFFFu;
@@ -664,7 +693,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     if constexpr (LAYERED) {
       const uint32_t L = p.lab;
       const float zlo = szb[2u * L], zhi = szb[2u * L + 1u];
-      const MatDev ml = mats[L];
+      const MatDev ml = load_mat(mats + L);
       flush_pending();
       if (in_sphere(m, p.x, p.y, p.z)) { /* :83-92 */
         const float mutr = mats[m.n_layers + 2u].mutr;
@@ -695,9 +724,9 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
         s = 0.0f;
         reflect<!(TEMPORAL && PERJOB)>(mats, p, nb, m.last, x, a);
       }
-      mm = in_sphere(m, p.x, p.y, p.z) ? mats[m.n_layers + 2u] : mats[p.lab]; /* :200-224 */
+      mm = load_mat(mats + (in_sphere(m, p.x, p.y, p.z) ? m.n_layers + 2u : p.lab)); /* :200-224 */
     } else {
-      const MatDev mc = mats[p.lab];
+      const MatDev mc = load_mat(mats + p.lab);
       flush_pending();
       if (mc.mutr != kFltMax) {
         s = -mcfl_log(mcfl_rand_oc(&x, a)) * mc.mutr;
@@ -745,7 +774,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       mm = mc; /* material after a possible interface crossing */
       if (nb != p.lab) {
         reflect<!(TEMPORAL && PERJOB)>(mats, p, nb, m.last, x, a);
-        mm = mats[p.lab];
+        mm = load_mat(mats + p.lab);
       }
     }
 
@@ -754,7 +783,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     if (p.lab == 0u || p.lab == m.last) {
       /* a job's image scale from its exit constants in LDS; a voxel emission job's per voxel */
       exit_tally<TEMPORAL>(A.jobs[job], sexit[job], p, p.lab == 0u, (flags & MCFL_F_SCALED) != 0u,
-                           WORK == MCFL_WORK_JOBS ? sexit[job].exit_scale : escale, st);
+                           JOBS ? sexit[job].exit_scale : escale, st);
       p.w = 0u;
       s = 0.0f;
     }
@@ -998,7 +1027,9 @@ const void* kfn() {
 /* perjob selects mixed-media launches (jobs only); layered the bulk-method-1
    medium (jobs and reference lanes, one medium per launch); wg the workgroup
    size (MCFL_WG, or MCFL_WG_ALT1/2 for non-layered job launches) */
-const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t wg) {
+const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t wg, bool one = false) {
+  if (one && work == MCFL_WORK_JOBS && !perjob && !layered && wg == MCFL_WG)
+    return temporal ? kfn<MCFL_WORK_ONE, true, false, false>() : kfn<MCFL_WORK_ONE, false, false, false>();
   if (wg != MCFL_WG) {
     if (work != MCFL_WORK_JOBS || layered) return nullptr;
     if (wg == MCFL_WG_ALT1) {
@@ -1057,7 +1088,7 @@ extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perj
   if (A->med.n_layers > 0 && !A->med.zb) return -1;
   const int layered = A->med.n_layers > 0u ? 1 : 0;
   const uint32_t wg = mcfl_wg_of(A);
-  const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered, wg);
+  const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered, wg, A->njobs == 1u);
   if (!k || (layered && perjob)) return -1;
   if (A->n_waves != (uint32_t)nblocks * (wg / 64u)) return -1;
   void* args[] = {const_cast<LaunchArgs*>(A)};
