@@ -91,6 +91,13 @@ typedef __attribute__((address_space(1))) const uint32_t gcu32;
 
 /* p[i] through a global (not generic) pointer: a global load instead of a flat one */
 __device__ __forceinline__ uint32_t gload_u32(const uint32_t* p, uint32_t i) { return ((gcu32*)p)[i]; }
+/* The job table does not change during a launch: read through the constant
+   address space, its wave-uniform reads are scalar loads, which wait neither
+   for nor behind the wave's outstanding vector memory operations (the
+   compiler cannot prove that the tally atomics leave generic job-table
+   memory untouched, and would issue vector loads). */
+typedef __attribute__((address_space(4))) const JobDev cjob;
+__device__ __forceinline__ cjob* ctab(const JobDev* p) { return (cjob*)p; }
 
 __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v) {
   __hip_atomic_fetch_add((gu64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -536,7 +543,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     while (needmask != 0ull) {
       if (pool_left == 0u) {
         if (pool_dead) break;
-        const uint64_t cnt = JOBS ? A.jobs[cur].count : A.total;
+        const uint64_t cnt = ONE ? ctab(A.jobs)[0].count : (JOBS ? A.jobs[cur].count : A.total);
         /* guided: the job's remainder (beyond the wave's last chunk) over
            four times the launch's waves, within [64, chunk_max] */
         const uint64_t rem = cnt > pool_next ? cnt - pool_next : 0ull;
@@ -574,33 +581,41 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
         accA = 0;
         iters = 0;
         if (JOBS) {
-          /* the job table entry through the wave-uniform cur: scalar loads,
-             which do not wait behind the wave's outstanding vector memory
-             operations (tally atomics) */
-          const JobDev& J = A.jobs[ONE ? 0u : cur];
-          const uint64_t k = g + J.k0;
-          a = A.a_tab[k & A.a_mask];
-          x = mcfl_photon_state(J.key, k, a);
-          if (!ONE) {
-            job = cur;
-            mbase = J.mat_base;
-            flags = J.flags;
-            fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
-            axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
-          }
-          if (J.src_kind == MCFL_SRC_EMISSION) {
-            /* importance-sampled emission voxel (Walker alias table), then an
-               isotropic photon from its corner as in CUDAMCFLmain.cu:481-487 */
-            const uint32_t col = (uint32_t)(((uint64_t)mcfl_mwc_next(&x, a) * (uint64_t)J.al_n) >> 32);
-            const uint32_t u = mcfl_mwc_next(&x, a);
-            const uint32_t v = gload_u32(J.al_vox, u < gload_u32(J.al_prob, col) ? col : gload_u32(J.al_alias, col));
-            const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
-            const uint32_t iz = v / nxy, rem = v - iz * nxy, iy = rem / (uint32_t)m.nx, ix = rem - iy * (uint32_t)m.nx;
-            launch(m, MCFL_SRC_ISOTROPIC, ((float)ix / m.gs) - m.halfx, ((float)iy / m.gs) - m.halfy,
-                   ((float)iz / m.gs), 0.0f, 0u, p, x, a, false, 0u);
-          } else {
-            launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a, LAYERED, J.start_lab);
-          }
+          /* the job's photon k: its stream, then its launch */
+          auto start = [&](auto& J) {
+            const uint64_t k = g + J.k0;
+            a = A.a_tab[k & A.a_mask];
+            x = mcfl_photon_state(J.key, k, a);
+            if (!ONE) {
+              job = cur;
+              mbase = J.mat_base;
+              flags = J.flags;
+              fhd = J.fhd ? J.fhd + xcc * J.copy_stride : nullptr;
+              axyz = J.axyz ? J.axyz + xcc * J.copy_stride : nullptr;
+            }
+            if (J.src_kind == MCFL_SRC_EMISSION) {
+              /* importance-sampled emission voxel (Walker alias table), then an
+                 isotropic photon from its corner as in CUDAMCFLmain.cu:481-487 */
+              const uint32_t col = (uint32_t)(((uint64_t)mcfl_mwc_next(&x, a) * (uint64_t)J.al_n) >> 32);
+              const uint32_t u = mcfl_mwc_next(&x, a);
+              const uint32_t v =
+                  gload_u32(J.al_vox, u < gload_u32(J.al_prob, col) ? col : gload_u32(J.al_alias, col));
+              const uint32_t nxy = (uint32_t)m.nx * (uint32_t)m.ny;
+              const uint32_t iz = v / nxy, rem = v - iz * nxy, iy = rem / (uint32_t)m.nx, ix = rem - iy * (uint32_t)m.nx;
+              launch(m, MCFL_SRC_ISOTROPIC, ((float)ix / m.gs) - m.halfx, ((float)iy / m.gs) - m.halfy,
+                     ((float)iz / m.gs), 0.0f, 0u, p, x, a, false, 0u);
+            } else {
+              launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a, LAYERED, J.start_lab);
+            }
+          };
+          /* one job: its entry through scalar loads, which do not wait for
+             the wave's outstanding vector memory operations (tally atomics);
+             several: the generic entry of the wave-uniform cur (the scalar
+             loads' registers spill in that variant, measured slower) */
+          if (ONE)
+            start(ctab(A.jobs)[0]);
+          else
+            start(A.jobs[cur]);
         } else if (WORK == MCFL_WORK_VOXELS) {
           const uint64_t j = g / A.per_job;
           const uint64_t k = g - j * A.per_job;
@@ -849,7 +864,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       alive = false;
       if (WORK == MCFL_WORK_LANES) {
         if (lane_left > 0ull) { /* next photon of the same lane, CUDAMCFLtransport.cu:437-440 */
-          const JobDev& J = A.jobs[0];
+          cjob& J = ctab(A.jobs)[0];
           --lane_left;
           accA = 0;
           iters = 0;
