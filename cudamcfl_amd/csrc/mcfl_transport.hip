@@ -456,7 +456,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
      and moves on to the other jobs when its home job is exhausted.  The
      workgroup's constants live in LDS (read on voxel changes and refills, so
      they hold no registers): home job, tile job (~0: no tile), tile origin. */
-  __shared__ uint32_t swg[4];
+  __shared__ __align__(16) uint32_t swg[8];
   if (tid == 0) {
     uint32_t home = 0;
     if (WORK == MCFL_WORK_JOBS && A.njobs > 1) {
@@ -469,6 +469,11 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     swg[1] = tiled ? home : 0xFFFFFFFFu;
     swg[2] = tiled ? HJ.tx0 : 0u;
     swg[3] = tiled ? HJ.ty0 : 0u;
+    /* the tile box, beside its origin (read on voxel changes by single-job launches) */
+    swg[4] = A.tnx;
+    swg[5] = A.tny;
+    swg[6] = A.tnz;
+    swg[7] = 0u;
   }
   __syncthreads();
 
@@ -514,7 +519,11 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
 
   /* the tile's job and origin as scalars (no LDS read on voxel changes) */
   const uint32_t tile_job = __builtin_amdgcn_readfirstlane(swg[1]);
-  const uint32_t tx0 = __builtin_amdgcn_readfirstlane(swg[2]), ty0 = __builtin_amdgcn_readfirstlane(swg[3]);
+  /* the tile origin in scalar registers (jobs launches), or read from LDS
+     where it is used (single-job launches, whose scalar registers are full
+     with the job's own values) */
+  const uint32_t tx0 = ONE ? 0u : __builtin_amdgcn_readfirstlane(swg[2]);
+  const uint32_t ty0 = ONE ? 0u : __builtin_amdgcn_readfirstlane(swg[3]);
   /* wave-uniform chunk of the current job's photons: [pool_next, pool_next + pool_left) */
   uint32_t cur = ONE ? 0u : __builtin_amdgcn_readfirstlane(swg[0]), pool_left = 0;
   /* PERJOB: the wave's photons share one medium (m stays wave-uniform), that
@@ -698,8 +707,12 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           facc = 0ull;
           ft = 0xFFFFFFFFu;
           if (job == tile_job) { /* inside the LDS tile of the workgroup's home job? */
-            const uint32_t lx = cl.ix - tx0, ly = cl.iy - ty0, lz = cl.iz;
-            if (lx < A.tnx && ly < A.tny && lz < A.tnz) ft = lx + __umul24(A.tnx, ly + __umul24(A.tny, lz));
+            typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+            typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+            const u2 o = ONE ? *reinterpret_cast<const u2*>(swg + 2) : u2{tx0, ty0}; /* [tx0, ty0] */
+            const u4 b = ONE ? *reinterpret_cast<const u4*>(swg + 4) : u4{A.tnx, A.tny, A.tnz, 0u}; /* [tnx, tny, tnz, 0] */
+            const uint32_t lx = cl.ix - o.x, ly = cl.iy - o.y, lz = cl.iz;
+            if (lx < b.x && ly < b.y && lz < b.z) ft = lx + __umul24(b.x, ly + __umul24(b.y, lz));
           }
         }
         facc += p.w;
