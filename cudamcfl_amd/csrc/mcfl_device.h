@@ -185,6 +185,10 @@ struct LaunchArgs {
   uint32_t refill_min; /* a wave refills when this many of its lanes are idle (1..64) */
   uint32_t wg;         /* threads per workgroup: 0 (MCFL_WG), or MCFL_WG_ALT1/2 for job launches */
   uint32_t max_steps;  /* per-photon step cap; 0: MCFL_MAX_STEPS_PER_PHOTON */
+  /* 1: no job of the launch tallies fhd or A_xyz (MCFL_F_FHD / MCFL_F_AXYZ),
+     and job launches run kernels compiled without the volume tallies; set
+     from the job table by whoever builds it (0, the default, is always safe) */
+  uint32_t novol;
 };
 
 /* Scratch of the allocation-free excitation (mcfl_run_excitation_noalloc):

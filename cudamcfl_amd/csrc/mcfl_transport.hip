@@ -409,7 +409,7 @@ __device__ __forceinline__ uint32_t job_at(const LaunchArgs& A, uint64_t g) {
    workgroup (4 waves per SIMD) is alone on its CU */
 constexpr int waves_per_eu_of(int wg) { return wg > 768 ? 4 : MCFL_WAVES_PER_EU; }
 
-template <int WORK, bool TEMPORAL, bool PERJOB, bool LAYERED, int WGS>
+template <int WORK, bool TEMPORAL, bool PERJOB, bool LAYERED, int WGS, bool VT>
 __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_eu_of(WGS)))) void mcfl_transport_kernel(LaunchArgs A) {
   /* dynamic LDS: [per-job totals | fhd tile | materials | layer bounds | per-job exit constants] */
   extern __shared__ __align__(16) unsigned long long lds[];
@@ -417,6 +417,10 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   /* JOBS: photons of the job table (ONE: a table of one job) */
   constexpr bool JOBS = WORK == MCFL_WORK_JOBS || WORK == MCFL_WORK_ONE;
   constexpr bool ONE = WORK == MCFL_WORK_ONE;
+  /* volume tallies (fhd, A_xyz): compiled in the VT variants of excitation
+     launches; the voxel-emitter launches of the fluorescence stage never
+     carry them */
+  constexpr bool VOL = VT && WORK != MCFL_WORK_VOXELS && WORK != MCFL_WORK_VLANES;
   const uint32_t tid = threadIdx.x;
   const uint32_t nslots = (WORK == MCFL_WORK_JOBS) ? A.njobs : 1u;
   unsigned long long* stot = lds;
@@ -680,7 +684,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     /* the previous step's flush, issued once the step's material is read (a
        relaunched photon's entry label may still be loading before that) */
     auto flush_pending = [&]() {
-      if (pa != 0ull) {
+      if (VOL && pa != 0ull) {
         atomic_add_u64(fhd + pv, pa);
         pa = 0ull;
       }
@@ -695,7 +699,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
     /* hitting density: the pre-absorption weight at the step's end point,
        summed per lane while the photon stays in one voxel (CUDAMCFLtransport.cu:358-366) */
     auto fhd_visit = [&]() {
-      if ((flags & MCFL_F_FHD) && in) {
+      if (VOL && (flags & MCFL_F_FHD) && in) {
         if (vi != fv) {
           /* the sum of the voxel left: into the tile, or pending (pa is 0
              here: the step's start flushed it) */
@@ -822,7 +826,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       if (LAYERED || w > 0u) {
         p.w -= w;
         accA += w;
-        if ((flags & MCFL_F_AXYZ) && in) {
+        if (VOL && (flags & MCFL_F_AXYZ) && in) {
           if (vi != av) {
             if (aacc != 0ull) atomic_add_u64(axyz + av, aacc);
             av = vi;
@@ -858,18 +862,18 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       lives = false;
     }
     if (!lives) {
-      if (facc != 0ull) {
+      if (VOL && facc != 0ull) {
         if (ft != 0xFFFFFFFFu)
           lds_add_u64(stile + ft, facc);
         else
           atomic_add_u64(fhd + fv, facc);
       }
       ft = 0xFFFFFFFFu;
-      if (pa != 0ull) {
+      if (VOL && pa != 0ull) {
         atomic_add_u64(fhd + pv, pa);
         pa = 0ull;
       }
-      if (aacc != 0ull) atomic_add_u64(axyz + av, aacc);
+      if (VOL && aacc != 0ull) atomic_add_u64(axyz + av, aacc);
       fv = av = 0xFFFFFFFFu;
       facc = aacc = 0ull;
       lds_add_u64(&st[MCFL_DT_A], accA);
@@ -1048,35 +1052,31 @@ __global__ void mcfl_math_eval_kernel(int fn, const uint32_t* in, uint32_t* out,
 /* ------------------------------------------------------------------ host */
 
 namespace {
-template <int W, bool T, bool P, bool L, int G = MCFL_WG>
+template <int W, bool T, bool P, bool L, int G = MCFL_WG, bool V = true>
 const void* kfn() {
-  return (const void*)mcfl_transport_kernel<W, T, P, L, G>;
+  return (const void*)mcfl_transport_kernel<W, T, P, L, G, V>;
+}
+/* job launches of workgroup size G: mixed media (P) or not, one job (ONE)
+   or several, with (V) or without volume tallies */
+template <int G, bool V>
+const void* job_kernel(int temporal, int perjob, bool one) {
+  if (one && !perjob && G == MCFL_WG)
+    return temporal ? kfn<MCFL_WORK_ONE, true, false, false, G, V>() : kfn<MCFL_WORK_ONE, false, false, false, G, V>();
+  if (perjob) return temporal ? kfn<MCFL_WORK_JOBS, true, true, false, G, V>() : kfn<MCFL_WORK_JOBS, false, true, false, G, V>();
+  return temporal ? kfn<MCFL_WORK_JOBS, true, false, false, G, V>() : kfn<MCFL_WORK_JOBS, false, false, false, G, V>();
+}
+template <int G>
+const void* job_kernel(int temporal, int perjob, bool one, bool novol) {
+  return novol ? job_kernel<G, false>(temporal, perjob, one) : job_kernel<G, true>(temporal, perjob, one);
 }
 /* perjob selects mixed-media launches (jobs only); layered the bulk-method-1
    medium (jobs and reference lanes, one medium per launch); wg the workgroup
-   size (MCFL_WG, or MCFL_WG_ALT1/2 for non-layered job launches) */
-const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t wg, bool one = false) {
-  if (one && work == MCFL_WORK_JOBS && !perjob && !layered && wg == MCFL_WG)
-    return temporal ? kfn<MCFL_WORK_ONE, true, false, false>() : kfn<MCFL_WORK_ONE, false, false, false>();
-  if (wg != MCFL_WG) {
-    if (work != MCFL_WORK_JOBS || layered) return nullptr;
-    if (wg == MCFL_WG_ALT1) {
-      if (perjob)
-        return temporal ? kfn<MCFL_WORK_JOBS, true, true, false, MCFL_WG_ALT1>()
-                        : kfn<MCFL_WORK_JOBS, false, true, false, MCFL_WG_ALT1>();
-      return temporal ? kfn<MCFL_WORK_JOBS, true, false, false, MCFL_WG_ALT1>()
-                      : kfn<MCFL_WORK_JOBS, false, false, false, MCFL_WG_ALT1>();
-    }
-    if (wg == MCFL_WG_ALT2) {
-      if (perjob)
-        return temporal ? kfn<MCFL_WORK_JOBS, true, true, false, MCFL_WG_ALT2>()
-                        : kfn<MCFL_WORK_JOBS, false, true, false, MCFL_WG_ALT2>();
-      return temporal ? kfn<MCFL_WORK_JOBS, true, false, false, MCFL_WG_ALT2>()
-                      : kfn<MCFL_WORK_JOBS, false, false, false, MCFL_WG_ALT2>();
-    }
-    return nullptr;
-  }
+   size (MCFL_WG, or MCFL_WG_ALT1/2 for non-layered job launches); one a job
+   launch of a single job; novol a job launch without volume tallies */
+const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t wg, bool one = false,
+                      bool novol = false) {
   if (layered) {
+    if (wg != MCFL_WG) return nullptr;
     if (work == MCFL_WORK_JOBS)
       return temporal ? kfn<MCFL_WORK_JOBS, true, false, true>() : kfn<MCFL_WORK_JOBS, false, false, true>();
     if (work == MCFL_WORK_LANES)
@@ -1084,10 +1084,12 @@ const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t 
     return nullptr;
   }
   if (work == MCFL_WORK_JOBS) {
-    if (perjob)
-      return temporal ? kfn<MCFL_WORK_JOBS, true, true, false>() : kfn<MCFL_WORK_JOBS, false, true, false>();
-    return temporal ? kfn<MCFL_WORK_JOBS, true, false, false>() : kfn<MCFL_WORK_JOBS, false, false, false>();
+    if (wg == MCFL_WG) return job_kernel<MCFL_WG>(temporal, perjob, one, novol);
+    if (wg == MCFL_WG_ALT1) return job_kernel<MCFL_WG_ALT1>(temporal, perjob, false, novol);
+    if (wg == MCFL_WG_ALT2) return job_kernel<MCFL_WG_ALT2>(temporal, perjob, false, novol);
+    return nullptr;
   }
+  if (wg != MCFL_WG) return nullptr;
   if (work == MCFL_WORK_VOXELS)
     return temporal ? kfn<MCFL_WORK_VOXELS, true, false, false>() : kfn<MCFL_WORK_VOXELS, false, false, false>();
   if (work == MCFL_WORK_VLANES) return temporal ? nullptr : kfn<MCFL_WORK_VLANES, false, false, false>();
@@ -1116,7 +1118,8 @@ extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perj
   if (A->med.n_layers > 0 && !A->med.zb) return -1;
   const int layered = A->med.n_layers > 0u ? 1 : 0;
   const uint32_t wg = mcfl_wg_of(A);
-  const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered, wg, A->njobs == 1u);
+  const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered, wg, A->njobs == 1u,
+                            A->novol != 0u);
   if (!k || (layered && perjob)) return -1;
   if (A->n_waves != (uint32_t)nblocks * (wg / 64u)) return -1;
   void* args[] = {const_cast<LaunchArgs*>(A)};
