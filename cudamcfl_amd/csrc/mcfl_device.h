@@ -41,6 +41,8 @@ enum { MCFL_WORK_JOBS = 0, MCFL_WORK_VOXELS = 1, MCFL_WORK_LANES = 2, MCFL_WORK_
 /* Kernel-internal: a MCFL_WORK_JOBS launch of one job (the job's tally
    pointers, flags and material base are then wave-uniform scalars). */
 enum { MCFL_WORK_ONE = 4 };
+/* Volume tallies compiled into a job kernel: fhd and A_xyz, fhd alone, none. */
+enum { MCFL_VOL_ALL = 0, MCFL_VOL_FHD = 1, MCFL_VOL_NONE = 2 };
 enum { MCFL_SRC_ISOTROPIC = 0, MCFL_SRC_PENCIL = 1, MCFL_SRC_DISK = 2, MCFL_SRC_EMISSION = 3 };
 enum {
   MCFL_DT_LAUNCHED_W = 0,
@@ -104,6 +106,10 @@ enum { MCFL_ME_LOG = 0, MCFL_ME_RSQRT = 1, MCFL_ME_RENORM = 2, MCFL_ME_SINCOS = 
 
 /* Tally flags */
 enum { MCFL_F_FHD = 1, MCFL_F_AXYZ = 2, MCFL_F_SQ = 4, MCFL_F_TEMP = 8, MCFL_F_SCALED = 16 };
+/* The volume tallies a set of job flags needs. */
+static inline uint32_t mcfl_vol_of(uint32_t flags) {
+  return (flags & MCFL_F_AXYZ) ? (uint32_t)MCFL_VOL_ALL : ((flags & MCFL_F_FHD) ? (uint32_t)MCFL_VOL_FHD : (uint32_t)MCFL_VOL_NONE);
+}
 
 /* What the reflectance / transmittance image tally of an exit reads, one
    per job, copied from the job table into LDS when a workgroup starts: an
@@ -185,10 +191,10 @@ struct LaunchArgs {
   uint32_t refill_min; /* a wave refills when this many of its lanes are idle (1..64) */
   uint32_t wg;         /* threads per workgroup: 0 (MCFL_WG), or MCFL_WG_ALT1/2 for job launches */
   uint32_t max_steps;  /* per-photon step cap; 0: MCFL_MAX_STEPS_PER_PHOTON */
-  /* 1: no job of the launch tallies fhd or A_xyz (MCFL_F_FHD / MCFL_F_AXYZ),
-     and job launches run kernels compiled without the volume tallies; set
-     from the job table by whoever builds it (0, the default, is always safe) */
-  uint32_t novol;
+  /* the volume tallies the launch's jobs need (MCFL_VOL_*): job launches run
+     kernels compiled with just those; set from the job table by whoever
+     builds it (MCFL_VOL_ALL, 0, is always safe) */
+  uint32_t vol;
 };
 
 /* Scratch of the allocation-free excitation (mcfl_run_excitation_noalloc):

@@ -445,13 +445,13 @@ struct Launcher {
     A->n_mats = (uint32_t)mt.mats.size();
     A->work = MCFL_WORK_JOBS;
     uint64_t total = 0;
-    uint32_t vol = 0;
+    uint32_t flags = 0;
     for (const JobDev& J : jd) {
       total += J.count;
-      vol |= J.flags & (MCFL_F_FHD | MCFL_F_AXYZ);
+      flags |= J.flags;
     }
     A->total = total;
-    A->novol = vol ? 0u : 1u;
+    A->vol = mcfl_vol_of(flags);
     A->counter = d_counter.as<unsigned long long>();
     A->a_tab = c->d_atab;
     A->a_mask = kTablePhotons - 1;

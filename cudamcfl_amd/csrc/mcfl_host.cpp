@@ -697,7 +697,7 @@ mcfl_status run_excitation(mcfl_ctx* c, const mcfl_sim* s, const mcfl_rng* rng, 
     A.work = MCFL_WORK_JOBS;
     A.total = N;
   }
-  A.novol = (J.flags & (MCFL_F_FHD | MCFL_F_AXYZ)) ? 0u : 1u;
+  A.vol = mcfl_vol_of(J.flags);
   if (fhd_on) {
     choose_tile(c, A, A.med, temporal, 0);
     place_tile(A, A.med, s->xi, s->yi, J);

@@ -409,7 +409,7 @@ __device__ __forceinline__ uint32_t job_at(const LaunchArgs& A, uint64_t g) {
    workgroup (4 waves per SIMD) is alone on its CU */
 constexpr int waves_per_eu_of(int wg) { return wg > 768 ? 4 : MCFL_WAVES_PER_EU; }
 
-template <int WORK, bool TEMPORAL, bool PERJOB, bool LAYERED, int WGS, bool VT>
+template <int WORK, bool TEMPORAL, bool PERJOB, bool LAYERED, int WGS, int VM>
 __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_eu_of(WGS)))) void mcfl_transport_kernel(LaunchArgs A) {
   /* dynamic LDS: [per-job totals | fhd tile | materials | layer bounds | per-job exit constants] */
   extern __shared__ __align__(16) unsigned long long lds[];
@@ -417,10 +417,12 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   /* JOBS: photons of the job table (ONE: a table of one job) */
   constexpr bool JOBS = WORK == MCFL_WORK_JOBS || WORK == MCFL_WORK_ONE;
   constexpr bool ONE = WORK == MCFL_WORK_ONE;
-  /* volume tallies (fhd, A_xyz): compiled in the VT variants of excitation
-     launches; the voxel-emitter launches of the fluorescence stage never
+  /* volume tallies: fhd and A_xyz (VM = MCFL_VOL_ALL), fhd alone
+     (MCFL_VOL_FHD) or none (MCFL_VOL_NONE) compiled in, as the launch's jobs
+     need them; the voxel-emitter launches of the fluorescence stage never
      carry them */
-  constexpr bool VOL = VT && WORK != MCFL_WORK_VOXELS && WORK != MCFL_WORK_VLANES;
+  constexpr bool VOL = VM != MCFL_VOL_NONE && WORK != MCFL_WORK_VOXELS && WORK != MCFL_WORK_VLANES;
+  constexpr bool AX = VOL && VM == MCFL_VOL_ALL;
   const uint32_t tid = threadIdx.x;
   const uint32_t nslots = (WORK == MCFL_WORK_JOBS) ? A.njobs : 1u;
   unsigned long long* stot = lds;
@@ -826,7 +828,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       if (LAYERED || w > 0u) {
         p.w -= w;
         accA += w;
-        if (VOL && (flags & MCFL_F_AXYZ) && in) {
+        if (AX && (flags & MCFL_F_AXYZ) && in) {
           if (vi != av) {
             if (aacc != 0ull) atomic_add_u64(axyz + av, aacc);
             av = vi;
@@ -873,7 +875,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
         atomic_add_u64(fhd + pv, pa);
         pa = 0ull;
       }
-      if (VOL && aacc != 0ull) atomic_add_u64(axyz + av, aacc);
+      if (AX && aacc != 0ull) atomic_add_u64(axyz + av, aacc);
       fv = av = 0xFFFFFFFFu;
       facc = aacc = 0ull;
       lds_add_u64(&st[MCFL_DT_A], accA);
@@ -1052,13 +1054,13 @@ __global__ void mcfl_math_eval_kernel(int fn, const uint32_t* in, uint32_t* out,
 /* ------------------------------------------------------------------ host */
 
 namespace {
-template <int W, bool T, bool P, bool L, int G = MCFL_WG, bool V = true>
+template <int W, bool T, bool P, bool L, int G = MCFL_WG, int V = MCFL_VOL_ALL>
 const void* kfn() {
   return (const void*)mcfl_transport_kernel<W, T, P, L, G, V>;
 }
 /* job launches of workgroup size G: mixed media (P) or not, one job (ONE)
-   or several, with (V) or without volume tallies */
-template <int G, bool V>
+   or several, with the volume tallies V */
+template <int G, int V>
 const void* job_kernel(int temporal, int perjob, bool one) {
   if (one && !perjob && G == MCFL_WG)
     return temporal ? kfn<MCFL_WORK_ONE, true, false, false, G, V>() : kfn<MCFL_WORK_ONE, false, false, false, G, V>();
@@ -1066,15 +1068,17 @@ const void* job_kernel(int temporal, int perjob, bool one) {
   return temporal ? kfn<MCFL_WORK_JOBS, true, false, false, G, V>() : kfn<MCFL_WORK_JOBS, false, false, false, G, V>();
 }
 template <int G>
-const void* job_kernel(int temporal, int perjob, bool one, bool novol) {
-  return novol ? job_kernel<G, false>(temporal, perjob, one) : job_kernel<G, true>(temporal, perjob, one);
+const void* job_kernel(int temporal, int perjob, bool one, uint32_t vol) {
+  if (vol == MCFL_VOL_NONE) return job_kernel<G, MCFL_VOL_NONE>(temporal, perjob, one);
+  if (vol == MCFL_VOL_FHD) return job_kernel<G, MCFL_VOL_FHD>(temporal, perjob, one);
+  return job_kernel<G, MCFL_VOL_ALL>(temporal, perjob, one);
 }
 /* perjob selects mixed-media launches (jobs only); layered the bulk-method-1
    medium (jobs and reference lanes, one medium per launch); wg the workgroup
    size (MCFL_WG, or MCFL_WG_ALT1/2 for non-layered job launches); one a job
-   launch of a single job; novol a job launch without volume tallies */
+   launch of a single job; vol the volume tallies of a job launch (MCFL_VOL_*) */
 const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t wg, bool one = false,
-                      bool novol = false) {
+                      uint32_t vol = MCFL_VOL_ALL) {
   if (layered) {
     if (wg != MCFL_WG) return nullptr;
     if (work == MCFL_WORK_JOBS)
@@ -1084,9 +1088,9 @@ const void* kernel_of(int work, int temporal, int perjob, int layered, uint32_t 
     return nullptr;
   }
   if (work == MCFL_WORK_JOBS) {
-    if (wg == MCFL_WG) return job_kernel<MCFL_WG>(temporal, perjob, one, novol);
-    if (wg == MCFL_WG_ALT1) return job_kernel<MCFL_WG_ALT1>(temporal, perjob, false, novol);
-    if (wg == MCFL_WG_ALT2) return job_kernel<MCFL_WG_ALT2>(temporal, perjob, false, novol);
+    if (wg == MCFL_WG) return job_kernel<MCFL_WG>(temporal, perjob, one, vol);
+    if (wg == MCFL_WG_ALT1) return job_kernel<MCFL_WG_ALT1>(temporal, perjob, false, vol);
+    if (wg == MCFL_WG_ALT2) return job_kernel<MCFL_WG_ALT2>(temporal, perjob, false, vol);
     return nullptr;
   }
   if (wg != MCFL_WG) return nullptr;
@@ -1119,7 +1123,7 @@ extern "C" int mcfl_transport_launch(const LaunchArgs* A, int temporal, int perj
   const int layered = A->med.n_layers > 0u ? 1 : 0;
   const uint32_t wg = mcfl_wg_of(A);
   const void* k = kernel_of(A->work, temporal, A->work == MCFL_WORK_JOBS ? perjob : 0, layered, wg, A->njobs == 1u,
-                            A->novol != 0u);
+                            A->vol);
   if (!k || (layered && perjob)) return -1;
   if (A->n_waves != (uint32_t)nblocks * (wg / 64u)) return -1;
   void* args[] = {const_cast<LaunchArgs*>(A)};
