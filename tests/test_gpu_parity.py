@@ -53,6 +53,22 @@ def test_photon_streams_bit_exact(ctx):
     assert g.Rd.sum() > 0 and g.Tt.sum() > 0 and g.time.sum() > 0
 
 
+@pytest.mark.parametrize("fhd,axyz", [(False, True), (True, False), (False, False)])
+def test_volume_tally_kernels_bit_exact(ctx, fhd, axyz):
+    """Launches compile only the volume tallies their jobs need (fhd and A_xyz,
+    fhd alone, none): each of those kernels against the oracle, including
+    A_xyz without fhd (which needs the full kernel)."""
+    sim = small_sim(photons=20000, fhd=fhd)
+    print(f"case: volume tallies fhd={fhd} axyz={axyz}, 20x20x10, 20000 photons, seed 8", flush=True)
+    g = ctx.run_excitation(sim, seed=8, axyz=axyz, sq=True)
+    o = O.FromProduct(sim, axyz=axyz, sq=True)
+    r = O.run_photons(o.o, a_tab(), M.stream_key(8, 0), 0, 20000)
+    assert_same(g, r)
+    check_totals(g.totals, r.tot)
+    assert (g.fhd is not None and g.fhd.sum() > 0) == fhd
+    assert (g.A_xyz is not None and g.A_xyz.sum() > 0) == axyz
+
+
 def test_lane_streams_bit_exact(ctx):
     sim = small_sim(photons=5003, temporal=False)
     x, a = M.init_rng(512, seed=12345)
