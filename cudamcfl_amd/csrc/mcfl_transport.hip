@@ -22,6 +22,9 @@
  *    per-step volume tallies (photon hitting density, absorbed weight) go to
  *    one copy per XCD, chosen by the hardware XCC_ID, so their cache lines
  *    never move between the XCDs' L2 caches; a reduce kernel sums the copies.
+ *  - Variants: the launch's shape is compiled in (one job or several, one
+ *    medium or several, time-of-flight tally or not, the volume tallies its
+ *    jobs carry), so a launch holds only the registers and code it uses.
  *
  * The step follows CUDAMCFLtransport.cu:289-447 quirk for quirk (end-point
  * label lookup, z-normal Fresnel, absorption and spin at a surface point after
