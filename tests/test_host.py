@@ -4,6 +4,7 @@ layouts, the emission plan, and the command-line front end."""
 import ctypes as C
 import os
 import re
+import shutil
 import subprocess
 import sys
 
@@ -360,6 +361,34 @@ def test_no_kernel_variant_spills(tmp_path):
     assert len(kernels) >= 20
     spills = {name: int(n) for name, _, n in kernels if int(n) != 0}
     assert not spills, spills
+
+
+def test_volume_tally_kernel_choice(tmp_path):
+    """A job launch runs the kernel compiled with the volume tallies its jobs
+    carry: A_xyz (with or without fhd) needs the full kernel, fhd alone the
+    fhd kernel, neither (whatever the other flags) the kernel without them
+    (mcfl_vol_of in the kernels' shared header, compiled here with g++)."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "vol.cpp"
+    src.write_text('''#include <stdio.h>
+#include "mcfl_device.h"
+int main() {
+  const unsigned f[] = {0u, MCFL_F_FHD, MCFL_F_AXYZ, MCFL_F_FHD | MCFL_F_AXYZ, MCFL_F_SQ | MCFL_F_TEMP | MCFL_F_SCALED,
+                        MCFL_F_FHD | MCFL_F_SQ, MCFL_F_AXYZ | MCFL_F_TEMP};
+  for (unsigned k = 0; k < sizeof f / sizeof f[0]; ++k) printf("%u ", mcfl_vol_of(f[k]));
+  printf("%d %d %d\\n", (int)MCFL_VOL_ALL, (int)MCFL_VOL_FHD, (int)MCFL_VOL_NONE);
+  return 0;
+}
+''')
+    exe = str(tmp_path / "vol")
+    r = subprocess.run(["g++", "-std=c++17", "-o", exe, str(src), "-I" + os.path.join(ROOT, "cudamcfl_amd", "csrc"),
+                        "-I" + os.path.join(ROOT, "include")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    v = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    ALL, FHD, NONE = v[-3:]
+    assert v[:-3] == [NONE, FHD, ALL, ALL, NONE, FHD, ALL]
+    assert ALL == 0  # the zero-initialised launch arguments compile every tally in
 
 
 def test_bench_refuses_more_than_one_gpu():
