@@ -332,7 +332,6 @@ def test_entry_points_refuse_null_arguments():
 def test_integration_example_compiles_and_links(tmp_path):
     """The C snippet that shows a maintainer how to replace the reference's
     main() compiles as C against include/mcfl.h and links against libmcfl.so."""
-    import shutil
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
