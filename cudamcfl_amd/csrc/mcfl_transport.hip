@@ -517,6 +517,12 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
   /* the lane's last label lookup (voxel, label); cleared for each photon, whose
      medium may differ from the last one's */
   uint32_t lv = 0xFFFFFFFFu, ll = 0u;
+  /* the material of the photon's label, kept from the step (or launch) that
+     set the label instead of read again at every step: voxel medium, no
+     time-of-flight tally, not both volume tallies (the variants with the
+     most live registers read it every step; measured 5% slower kept) */
+  constexpr bool KEEPMAT = !LAYERED && !TEMPORAL && (!VOL || VM != MCFL_VOL_ALL);
+  MatDev mcur{};
   uint32_t iters = 0;
   bool alive = false, retired = false;
   uint32_t lane_item = 0;
@@ -670,6 +676,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, xl, a, LAYERED, J.start_lab);
         }
         lv = 0xFFFFFFFFu;
+        if (KEEPMAT) mcur = load_mat(smat + mbase + p.lab);
         unsigned long long* st = stot + job * MCFL_NTOT;
         lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
         lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
@@ -763,7 +770,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       }
       mm = load_mat(mats + (in_sphere(m, p.x, p.y, p.z) ? m.n_layers + 2u : p.lab)); /* :200-224 */
     } else {
-      const MatDev mc = load_mat(mats + p.lab);
+      const MatDev mc = KEEPMAT ? mcur : load_mat(mats + p.lab);
       flush_pending();
       if (mc.mutr != kFltMax) {
         s = -mcfl_log(mcfl_rand_oc(&x, a)) * mc.mutr;
@@ -812,6 +819,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
       if (nb != p.lab) {
         reflect<!(TEMPORAL && PERJOB)>(mats, p, nb, m.last, x, a);
         mm = load_mat(mats + p.lab);
+        if (KEEPMAT) mcur = mm;
       }
     }
 
@@ -891,6 +899,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           accA = 0;
           iters = 0;
           launch(m, J.src_kind, J.xi, J.yi, J.zi, J.src_r, J.start_weight, p, x, a, LAYERED, J.start_lab);
+          if (KEEPMAT) mcur = load_mat(mats + p.lab);
           lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
           lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
           alive = true;
@@ -920,6 +929,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(waves_per_e
           } else {
             launch_voxel(m, A.vox[vj], p, x, a);
           }
+          if (KEEPMAT) mcur = load_mat(mats + p.lab);
           lds_add_u64(&st[MCFL_DT_PHOTONS], 1ull);
           lds_add_u64(&st[MCFL_DT_LAUNCHED_W], p.w);
           alive = true;
