@@ -1075,8 +1075,9 @@ const void* kfn() {
    or several, with the volume tallies V */
 template <int G, int V>
 const void* job_kernel(int temporal, int perjob, bool one) {
-  if (one && !perjob && G == MCFL_WG)
-    return temporal ? kfn<MCFL_WORK_ONE, true, false, false, G, V>() : kfn<MCFL_WORK_ONE, false, false, false, G, V>();
+  if constexpr (G == MCFL_WG) /* single-job launches run at the default size only */
+    if (one && !perjob)
+      return temporal ? kfn<MCFL_WORK_ONE, true, false, false, G, V>() : kfn<MCFL_WORK_ONE, false, false, false, G, V>();
   if (perjob) return temporal ? kfn<MCFL_WORK_JOBS, true, true, false, G, V>() : kfn<MCFL_WORK_JOBS, false, true, false, G, V>();
   return temporal ? kfn<MCFL_WORK_JOBS, true, false, false, G, V>() : kfn<MCFL_WORK_JOBS, false, false, false, G, V>();
 }
