@@ -138,12 +138,12 @@ MCFL_HD void mcfl_sincos_quarter(float y, float* sn, float* cs) {
   pc = mcfl_fma(pc, r2, 0.25366920232772827f);
   pc = mcfl_fma(pc, r2, -1.2337005138397217f);
   pc = mcfl_fma(pc, r2, 1.0f);
-  float s = (q & 1) ? pc : ps;
-  float c = (q & 1) ? ps : pc;
-  if (q & 2) s = -s;
-  if ((q + 1) & 2) c = -c;
-  *sn = s;
-  *cs = c;
+  const float s = (q & 1) ? pc : ps;
+  const float c = (q & 1) ? ps : pc;
+  /* the quadrant's signs as sign-bit flips: sin negative in quadrants 2, 3;
+     cos in 1, 2 */
+  *sn = mcfl_bitsf(mcfl_fbits(s) ^ ((uint32_t)(q & 2) << 30));
+  *cs = mcfl_bitsf(mcfl_fbits(c) ^ ((uint32_t)((q + 1) & 2) << 30));
 }
 
 /* sin(2*pi*t) and cos(2*pi*t). */
