@@ -156,6 +156,15 @@ def test_server_admission_by_footprint(ctx):
         if len(done) == 3:
             break
     assert sorted(done) == sorted(ids)
+    # one request per epoch: the epochs run single-job launches, whose results
+    # must still equal the request run alone
+    sim.s.number_of_photons = 1500
+    sim.s.xi, sim.s.yi, sim.s.zi = 0.0, 0.0, 0.0  # the requests' source
+    for seed, rid in enumerate(ids):
+        out = srv.collect(rid, mid)
+        r = ctx.run_excitation(sim, seed=seed)
+        assert np.array_equal(out["Rd"], r.Rd) and np.array_equal(out["Tt"], r.Tt), seed
+        assert out["totals"] == r.totals, seed
     with pytest.raises(M.McflError):
         srv.submit(mid, 100, 1, fluorescence=True)  # its footprint exceeds the whole budget
     srv.close()
